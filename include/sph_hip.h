@@ -1,0 +1,180 @@
+/*
+ * sph_hip.h -- C ABI of the MI355X-native SPH step (libsph_hip.so).
+ *
+ * Drop-in for the per-step operator of lijenicol/SPH-Fluid-Simulator:
+ *   void updateParticles(Particle*, glm::mat4*, size_t, const SPHSettings&,
+ *                        float deltaTime, bool onGPU)      src/sph.h:19-22
+ *   void updateParticlesGPU(Particle*, glm::mat4*, size_t,
+ *                           const SPHSettings&, float)     src/kernels/sphGPU.h:8-11
+ * with the semantics of the reference's multithreaded CPU path
+ * (updateParticlesCPU, src/sph.cpp:195-275), plus a resident API that keeps
+ * scene state in HBM across steps.
+ *
+ * Conventions: every entry point returns SPH_OK (0) or a negative SPH_E_*
+ * code and never throws; sph_hip_last_error() returns the calling thread's
+ * last message.  Plain pointers and sizes only.  Calls on one context must be
+ * serialised by the caller; distinct contexts are independent.  Host arrays
+ * of positions/velocities are n x 3 floats, row-major.
+ */
+#ifndef SPH_HIP_H
+#define SPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_HIP_ABI_VERSION 1
+
+#define SPH_OK 0
+#define SPH_E_INVALID -1   /* bad argument */
+#define SPH_E_NODEVICE -2  /* no usable HIP device */
+#define SPH_E_NOMEM -3     /* device or host allocation failed */
+#define SPH_E_HIP -4       /* other HIP runtime error (see last_error) */
+#define SPH_E_LIMIT -5     /* size beyond the supported maximum */
+#define SPH_E_STATE -6     /* call not valid in the object's current state */
+
+/* Mirrors struct Particle, src/Particle.h:4-10: glm::vec3 position, velocity,
+ * acceleration, force; float density, pressure; uint16_t hash; 2 bytes of
+ * tail padding.  sizeof == 60, alignment 4. */
+typedef struct sph_particle_v1 {
+    float position[3];
+    float velocity[3];
+    float acceleration[3];
+    float force[3];
+    float density;
+    float pressure;
+    uint16_t hash;
+    uint16_t _pad;
+} sph_particle_v1;
+
+/* Mirrors struct SPHSettings, src/SPHSystem.h:13-22 (glm::mat4 first, then
+ * 13 floats in declaration order).  sizeof == 116. */
+typedef struct sph_settings_v1 {
+    float sphereScale[16];
+    float poly6, spikyGrad, spikyLap, gasConstant, mass, h2, selfDens,
+        restDensity, viscosity, h, g, tension, massPoly6Product;
+} sph_settings_v1;
+
+/* Axis-aligned wall box.  Particles reflect off the planes lo + h and hi - h
+ * (floor, ceiling, x and z walls) with the given restitution and offset, in
+ * the order floor, ceiling, x-low, x-high, z-low, z-high.  An infinite bound
+ * disables that wall.  No counterpart struct in the reference: its CPU path
+ * hard-codes lo = (-8, 0, -8), hi = (8, +inf, 8), 0.5, 1e-4
+ * (src/sph.cpp:139-176), which sph_box_reference_cpu() reproduces. */
+typedef struct sph_box_v1 {
+    float lo[3];
+    float hi[3];
+    float restitution;
+    float offset;
+} sph_box_v1;
+
+/* One scene for the resident API. */
+typedef struct sph_scene_desc_v1 {
+    uint32_t n;               /* particle count (> 0) */
+    float dt;                 /* time step used by sph_hip_step */
+    sph_settings_v1 settings; /* from sph_settings_init */
+    sph_box_v1 box;
+    float cell_scale;         /* grid cell edge in units of h; 0 or 1 = 1 (min) */
+    uint32_t flags;           /* SPH_SCENE_* bits, others must be 0 */
+} sph_scene_desc_v1;
+
+/* Keep the per-particle force of the last step in device memory so that
+ * sph_hip_download_particles can return it (costs 16 bytes per particle
+ * per step of extra traffic). */
+#define SPH_SCENE_KEEP_FORCE 1u
+
+/* ---- host-only helpers (no device work) --------------------------------- */
+
+int sph_abi_version(void);
+const char *sph_hip_last_error(void);
+
+/* SPHSettings::SPHSettings(mass, restDensity, gasConst, viscosity, h, g,
+ * tension), src/SPHSystem.cpp:8-26: same derived constants, bit for bit. */
+int sph_settings_init(sph_settings_v1 *s, float mass, float restDensity,
+                      float gasConst, float viscosity, float h, float g,
+                      float tension);
+
+/* Fixed box of the reference CPU path, src/sph.cpp:139-176. */
+int sph_box_reference_cpu(sph_box_v1 *b);
+/* Fixed box of the reference CUDA path, src/kernels/sphGPU.cu:8-10,210-238
+ * (half-width 10, restitution 1, offset 1e-5). */
+int sph_box_reference_gpu(sph_box_v1 *b);
+
+/* getHash(ivec3), src/neighborTable.cpp:5-12 (18-bit value). */
+uint32_t sph_get_hash(int32_t cx, int32_t cy, int32_t cz);
+
+/* Host evaluation of the per-particle classification the kernels use:
+ * out[0] = 16-bit bucket (Particle::hash), out[1] = 1 if the particle takes
+ * the exact multiplicity path, out[2..4] = reference cell (getCell,
+ * src/neighborTable.cpp:14-17).  Exposed for tests. */
+int sph_classify_host(const float pos[3], float h, int32_t out[5]);
+
+/* ---- stateless drop-in (replaces updateParticles(..., onGPU)) -----------
+ * One step of the reference CPU operator on n particles, in place:
+ * particles come back sorted by hash (ties in input order); hash, density,
+ * pressure and force hold the values computed from the incoming positions;
+ * position and velocity are integrated; acceleration is carried through
+ * unchanged; transforms[i] (16 floats, column-major glm::mat4, may be NULL)
+ * is translate(position) * scale(h / 2) of the returned particles[i].
+ * The reference CPU wall box is used; the _box variant takes any box.
+ * n == 0 is a no-op.  Device buffers are cached between calls. */
+int sph_hip_update_particles(sph_particle_v1 *particles, float *transforms,
+                             size_t n, const sph_settings_v1 *settings,
+                             float dt);
+int sph_hip_update_particles_box(sph_particle_v1 *particles, float *transforms,
+                                 size_t n, const sph_settings_v1 *settings,
+                                 float dt, const sph_box_v1 *box);
+
+/* Neighbour multisets of the reference scan for the given particles (the
+ * sets parallelDensityAndPressures sums over, src/sph.cpp:40-66), computed
+ * by the density kernel's own candidate walk.  offsets has n + 1 entries;
+ * row i lists input indices j, ascending, each repeated m_ij times.  If idx
+ * is NULL or cap < total, only offsets and *total are written. */
+int sph_hip_neighbors(const sph_particle_v1 *particles, size_t n,
+                      const sph_settings_v1 *settings, uint64_t *offsets,
+                      uint32_t *idx, size_t cap, size_t *total);
+
+/* ---- resident single-scene API ------------------------------------------ */
+
+typedef struct sph_ctx sph_ctx;
+
+int sph_hip_create(const sph_scene_desc_v1 *desc, sph_ctx **out);
+int sph_hip_destroy(sph_ctx *ctx);
+/* Uploads n positions and velocities (vel may be NULL = zero); particle ids
+ * are 0..n-1 in upload order. */
+int sph_hip_upload(sph_ctx *ctx, const float *pos, const float *vel);
+/* Advances nsteps steps of dt on the context's stream (asynchronous). */
+int sph_hip_step(sph_ctx *ctx, int nsteps);
+int sph_hip_sync(sph_ctx *ctx);
+/* Copies the current state in device order; any pointer may be NULL.
+ * rho, pressure are the values computed during the last step (from the
+ * positions at its start).  Synchronises. */
+int sph_hip_download(sph_ctx *ctx, float *pos, float *vel, float *rho,
+                     float *pressure, uint32_t *ids);
+/* AoS view of the current state (reference Particle layout) plus the
+ * transform buffer the instanced renderer reads (16 floats per particle,
+ * may be NULL).  hash/density/pressure/force are from the last step. */
+int sph_hip_download_particles(sph_ctx *ctx, sph_particle_v1 *out,
+                               float *transforms);
+/* 1 = replay steps through a captured HIP graph (default), 0 = launch. */
+int sph_hip_set_graph(sph_ctx *ctx, int enable);
+uint32_t sph_hip_count(const sph_ctx *ctx);
+
+/* ---- measurement ---------------------------------------------------------
+ * Runs nsteps un-graphed steps with HIP events around every kernel on the
+ * context's stream.  names: nul-separated list written into names_buf;
+ * ms[k]: mean duration of kernel k per step; *nk: number of kernels. */
+int sph_hip_profile(sph_ctx *ctx, int nsteps, double *ms, int max_k, int *nk,
+                    char *names_buf, size_t names_cap);
+/* Device-to-device copy kernel over `bytes` bytes, `iters` times; *ms is
+ * the mean kernel time.  Calibrates bandwidth and HBM counters. */
+int sph_hip_copy_bench(size_t bytes, int iters, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_HIP_H */

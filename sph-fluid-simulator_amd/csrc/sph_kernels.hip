@@ -1,0 +1,744 @@
+// sph_kernels.hip -- hand-written CDNA4 (gfx950) kernels of one SPH step.
+//
+// One step (single scene) is, in launch order:
+//   scan          exclusive scan of per-cell counts -> cstart, counts := 0
+//   scatter       perm[cstart[cell] + rank] = A index
+//   reorder       stable cell order (ties by A index), gather A -> B
+//   density       poly6 density + pressure        (src/sph.cpp:28-76)
+//   force_integrate
+//                 spiky pressure + viscosity force (src/sph.cpp:80-129),
+//                 symplectic Euler + walls          (src/sph.cpp:133-181),
+//                 then the next step's cell key and in-cell rank
+// The hash / sort / table passes of the reference (src/sph.cpp:17-24,
+// 184-192, src/neighborTable.cpp:19-37) are replaced by the dense-grid
+// counting sort; the reference's 16-bit bucket semantics are kept through
+// the multiplicity rule documented in sph_common.h.
+//
+// Floating-point contraction is off for the whole file: the squared
+// distance must be fl(fl(fl(dx*dx) + fl(dy*dy)) + fl(dz*dz)) as glm forms it,
+// and integration reproduces the reference's rounding step by step.  The
+// force pass uses explicit fmaf() where the reference's rounding is not the
+// contract (tolerance-checked sums).
+#include "sph_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace sph {
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// Re-map so each XCD walks a contiguous range of particles: neighbouring
+// cells then share one XCD's L2.
+__device__ inline uint32_t xcd_block()
+{
+    uint32_t b = blockIdx.x, nb = gridDim.x;
+    uint32_t per = nb >> 3, rem = nb & 7u;
+    uint32_t xcd = b & 7u, k = b >> 3;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+__device__ inline uint32_t particle_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// ---------------------------------------------------------------- utilities
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t value, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        p[i] = value;
+}
+
+void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t value, size_t n)
+{
+    if (n == 0)
+        return;
+    size_t nb = (n + kBlock - 1) / kBlock;
+    if (nb > 4096)
+        nb = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)nb), dim3(kBlock), 0, st, p, value, n);
+}
+
+// Reference Particle AoS (60 bytes) -> posA/velA with id = input index.
+__global__ void k_unpack_aos(const float *__restrict__ aos, uint32_t n, float4 *__restrict__ posA,
+                             float4 *__restrict__ velA)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float *p = aos + (size_t)i * 15;
+    posA[i] = make_float4(p[0], p[1], p[2], 0.0f);
+    velA[i] = make_float4(p[3], p[4], p[5], __uint_as_float(i));
+}
+
+void launch_unpack_aos(hipStream_t st, const void *aos, uint32_t n, float4 *posA, float4 *velA)
+{
+    hipLaunchKernelGGL(k_unpack_aos, dim3(blocks_for(n)), dim3(kBlock), 0, st,
+                       (const float *)aos, n, posA, velA);
+}
+
+// Order-preserving float <-> int map for atomic min/max.
+__device__ inline int32_t ordered_int(float f)
+{
+    int32_t i = __float_as_int(f);
+    return i >= 0 ? i : (int32_t)(i ^ 0x7FFFFFFF);
+}
+
+// Bounding box of finite positions: bbox6 = {min x,y,z, max x,y,z} as
+// ordered ints (pre-filled with INT_MAX / INT_MIN).
+__global__ void k_bbox(const float4 *__restrict__ pos, uint32_t n, int32_t *__restrict__ bbox6)
+{
+    __shared__ int32_t red[6][kBlock / 64];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t v[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
+    if (i < n) {
+        float4 p = pos[i];
+        float c[3] = {p.x, p.y, p.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (c[k] - c[k] == 0.0f) {  // finite
+                int32_t o = ordered_int(c[k]);
+                v[k] = o;
+                v[k + 3] = o;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int32_t x = v[k];
+        for (int d = 32; d >= 1; d >>= 1) {
+            int32_t y = __shfl_xor(x, d, 64);
+            x = (k < 3) ? min(x, y) : max(x, y);
+        }
+        if ((threadIdx.x & 63) == 0)
+            red[k][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        int32_t x = red[k][0];
+        for (int w = 1; w < kBlock / 64; ++w)
+            x = (k < 3) ? min(x, red[k][w]) : max(x, red[k][w]);
+        if (k < 3)
+            atomicMin(&bbox6[k], x);
+        else
+            atomicMax(&bbox6[k], x);
+    }
+}
+
+void launch_bbox(hipStream_t st, const float4 *pos, uint32_t n, int32_t *bbox6)
+{
+    hipLaunchKernelGGL(k_bbox, dim3(blocks_for(n)), dim3(kBlock), 0, st, pos, n, bbox6);
+}
+
+// Cell key, bucket and in-cell arrival rank of the positions in posA.
+__global__ void k_init_keys(float4 *__restrict__ posA, uint32_t *__restrict__ cellcA,
+                            uint32_t *__restrict__ rankA, uint32_t *__restrict__ counts, uint32_t n,
+                            StepParams P)
+{
+    uint32_t i = particle_index();
+    if (i >= n)
+        return;
+    float4 p = posA[i];
+    Classified c = classify(p.x, p.y, p.z, P);
+    posA[i].w = __uint_as_float(c.h16);
+    cellcA[i] = c.cellc;
+    rankA[i] = atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
+}
+
+void launch_init_keys(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
+                      const StepParams &P)
+{
+    hipLaunchKernelGGL(k_init_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.posA, a.cellcA,
+                       a.rankA, c.counts, n, P);
+}
+
+// ------------------------------------------------------------- cell scan
+// Three passes over the (padded) cell array: per-block totals, a one-block
+// scan of the totals, then the block-local exclusive scan plus offset.  The
+// last pass also clears the counts for the next step's atomics.
+
+__device__ inline uint32_t wave_inclusive_scan(uint32_t x)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d)
+            x += y;
+    }
+    return x;
+}
+
+// Exclusive scan across the block of one value per thread; returns the
+// exclusive prefix and writes the block total.
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t *total)
+{
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_scan(x);
+    if (lane == 63)
+        wsum[wave] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        uint32_t s = wsum[w];
+        off += (w < wave) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - x;
+}
+
+__global__ void k_scan_reduce(const uint32_t *__restrict__ counts, uint32_t *__restrict__ partials)
+{
+    const uint4 *src = (const uint4 *)(counts + (size_t)blockIdx.x * kScanChunk);
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+        uint4 v = src[k * kBlock + threadIdx.x];
+        s += v.x + v.y + v.z + v.w;
+    }
+    uint32_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = tot;
+}
+
+// One block: exclusive scan of nparts block totals in place; cstart_end
+// receives the grand total.
+__global__ void k_scan_partials(uint32_t *__restrict__ partials, uint32_t nparts,
+                                uint32_t *__restrict__ cstart_end)
+{
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nparts; base += kBlock) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t x = i < nparts ? partials[i] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(x, &tot);
+        if (i < nparts)
+            partials[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0)
+        *cstart_end = carry;
+}
+
+__global__ void k_scan_final(uint32_t *__restrict__ counts, const uint32_t *__restrict__ partials,
+                             uint32_t *__restrict__ cstart)
+{
+    size_t base = (size_t)blockIdx.x * kScanChunk + (size_t)threadIdx.x * kScanItems;
+    uint4 *src = (uint4 *)(counts + base);
+    uint32_t v[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+        uint4 q = src[k];
+        v[4 * k + 0] = q.x;
+        v[4 * k + 1] = q.y;
+        v[4 * k + 2] = q.z;
+        v[4 * k + 3] = q.w;
+    }
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        local += v[k];
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(local, &tot) + partials[blockIdx.x];
+    uint4 *dst = (uint4 *)(cstart + base);
+#pragma unroll
+    for (int k = 0; k < kScanItems / 4; ++k) {
+        uint4 q;
+        q.x = run;
+        run += v[4 * k + 0];
+        q.y = run;
+        run += v[4 * k + 1];
+        q.z = run;
+        run += v[4 * k + 2];
+        q.w = run;
+        run += v[4 * k + 3];
+        dst[k] = q;
+        src[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+void launch_scan(hipStream_t st, const CellArrays &c, uint32_t /*n_total*/)
+{
+    uint32_t nparts = c.ncells_padded / kScanChunk;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kBlock), 0, st, c.counts, c.partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, st, c.partials, nparts,
+                       c.cstart + c.ncells_padded);
+    hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, st, c.counts, c.partials,
+                       c.cstart);
+}
+
+// ---------------------------------------------------------- scatter/reorder
+
+__global__ void k_scatter(const uint32_t *__restrict__ cellcA, const uint32_t *__restrict__ rankA,
+                          const uint32_t *__restrict__ cstart, uint32_t *__restrict__ perm,
+                          uint32_t n, StepParams P)
+{
+    uint32_t i = particle_index();
+    if (i >= n)
+        return;
+    perm[cstart[linear_cell(P, cellcA[i])] + rankA[i]] = i;
+}
+
+void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
+                    const StepParams &P)
+{
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.cellcA, a.rankA,
+                       c.cstart, a.perm, n, P);
+}
+
+// Slot s holds A index i in arrival order; its stable destination is the
+// cell start plus the number of smaller A indices in the same cell.
+__global__ void k_reorder(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ cellcA,
+                          const uint32_t *__restrict__ cstart, const float4 *__restrict__ posA,
+                          const float4 *__restrict__ velA, float4 *__restrict__ posB,
+                          float4 *__restrict__ velB, uint32_t *__restrict__ cellcB, uint32_t n,
+                          StepParams P)
+{
+    uint32_t s = particle_index();
+    if (s >= n)
+        return;
+    uint32_t i = perm[s];
+    uint32_t c = cellcA[i];
+    uint32_t lin = linear_cell(P, c);
+    uint32_t b = cstart[lin], e = cstart[lin + 1];
+    uint32_t d;
+    if (e - b <= kStableCellMax) {
+        uint32_t r = 0;
+        for (uint32_t t = b; t < e; ++t)
+            r += (perm[t] < i) ? 1u : 0u;
+        d = b + r;
+    } else {
+        d = s;
+    }
+    posB[d] = posA[i];
+    velB[d] = velA[i];
+    cellcB[d] = c;
+}
+
+void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
+                    const StepParams &P)
+{
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.perm, a.cellcA,
+                       c.cstart, a.posA, a.velA, a.posB, a.velB, a.cellcB, n, P);
+}
+
+// ------------------------------------------------------------ stencil walk
+// Calls body(j, q) for every candidate j != s in the 3x3x3 grid-cell block
+// around the particle; the x-run of three cells in each of the 9 rows is one
+// contiguous range of the cell-ordered arrays.
+template <class Body>
+__device__ inline void for_each_candidate(const float4 *__restrict__ pos,
+                                          const uint32_t *__restrict__ cstart, uint32_t s,
+                                          uint32_t cc, const StepParams &P, Body body)
+{
+    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc);
+    const int x0 = gx > 0 ? gx - 1 : 0;
+    const int x1 = gx + 1 < P.nx ? gx + 1 : P.nx - 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = gz + dz;
+        if (z < 0 || z >= P.nz)
+            continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = gy + dy;
+            if (y < 0 || y >= P.ny)
+                continue;
+            const uint32_t row = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
+            const uint32_t b = cstart[row + (uint32_t)x0];
+            const uint32_t e = cstart[row + (uint32_t)x1 + 1u];
+            for (uint32_t j = b; j < e; ++j) {
+                if (j == s)
+                    continue;
+                body(j, pos[j]);
+            }
+        }
+    }
+}
+
+// fl(fl(fl(dx*dx) + fl(dy*dy)) + fl(dz*dz)), dx = x_j - x_i: glm::length2.
+__device__ inline float dist2_ref(float4 pi, float4 q, float &dx, float &dy, float &dz)
+{
+    dx = q.x - pi.x;
+    dy = q.y - pi.y;
+    dz = q.z - pi.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+struct RefCell {
+    int x, y, z;
+};
+
+__device__ inline RefCell ref_cell(float4 p, const StepParams &P)
+{
+    return RefCell{trunc_to_int(p.x / P.h), trunc_to_int(p.y / P.h), trunc_to_int(p.z / P.h)};
+}
+
+// --------------------------------------------------------------- density
+// MODE 0: density + pressure.  MODE 1: also the neighbour count per
+// particle.  MODE 2: also the neighbour ids (multiplicity-repeated).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_density(
+    const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+    const uint32_t *__restrict__ cellc, const uint32_t *__restrict__ cstart,
+    float *__restrict__ rho, float2 *__restrict__ pinv, uint32_t n, StepParams P,
+    NeighborDump dump)
+{
+    uint32_t s = particle_index();
+    if (s >= n)
+        return;
+    const float4 pi = pos[s];
+    const uint32_t cc = cellc[s];
+    const bool slow = (cc & kSlowBit) != 0u;
+    const RefCell ci = ref_cell(pi, P);
+    float acc = 0.0f;
+    uint32_t cnt = 0;
+    uint64_t w = (MODE == 2) ? dump.off[s] : 0;
+    for_each_candidate(pos, cstart, s, cc, P, [&](uint32_t j, float4 q) {
+        float dx, dy, dz;
+        float d2 = dist2_ref(pi, q, dx, dy, dz);
+        if (d2 < P.h2) {
+            int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
+            float t = P.h2 - d2;
+            float term = P.mp6 * ((t * t) * t);
+            acc += (float)m * term;
+            if (MODE >= 1)
+                cnt += (uint32_t)m;
+            if (MODE == 2) {
+                uint32_t id = __float_as_uint(vel[j].w);
+                for (int k = 0; k < m; ++k)
+                    dump.out[w++] = id;
+            }
+        }
+    });
+    const float r = acc + P.self_dens;
+    rho[s] = r;
+    pinv[s] = make_float2(P.gas * (r - P.rho0), 1.0f / r);
+    if (MODE >= 1)
+        dump.count[s] = cnt;
+}
+
+void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
+                    const StepParams &P, const NeighborDump *dump)
+{
+    NeighborDump d = dump ? *dump : NeighborDump{nullptr, nullptr, nullptr};
+    dim3 g(blocks_for(n)), b(kBlock);
+    if (!dump)
+        hipLaunchKernelGGL(k_density<0>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
+                           a.pinvB, n, P, d);
+    else if (!dump->out)
+        hipLaunchKernelGGL(k_density<1>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
+                           a.pinvB, n, P, d);
+    else
+        hipLaunchKernelGGL(k_density<2>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
+                           a.pinvB, n, P, d);
+}
+
+// ----------------------------------------------------- force + integrate
+// Per neighbour (reference, src/sph.cpp:103-118):
+//   pressure = -dir m (p_i + p_j) / (2 rho_j) spikyGrad (h - r)^2
+//   visco    = mu m (v_j - v_i) / rho_j spikyLap (h - r)
+// with dir = (x_j - x_i) / r.  Both are evaluated as one scalar coefficient
+// times a vector, from 1 / rho_j precomputed by the density pass.
+__device__ inline void integrate_and_walls(float4 &x, float4 &v, float fx, float fy, float fz,
+                                           float rho, const StepParams &P)
+{
+    // src/sph.cpp:145-150: a = F / rho + (0, g, 0); v += a dt; x += v dt
+    float ax = fx / rho + 0.0f;
+    float ay = fy / rho + P.g;
+    float az = fz / rho + 0.0f;
+    v.x += ax * P.dt;
+    v.y += ay * P.dt;
+    v.z += az * P.dt;
+    x.x += v.x * P.dt;
+    x.y += v.y * P.dt;
+    x.z += v.z * P.dt;
+    // src/sph.cpp:153-176, planes L = lo + h, U = hi - h: floor, ceiling,
+    // x-low, x-high, z-low, z-high.
+    const float e = P.restitution, off = P.offset;
+    if (x.y < P.wall_lo[1]) {
+        x.y = -x.y + 2.0f * P.wall_lo[1] + off;
+        v.y = -v.y * e;
+    }
+    if (x.y > P.wall_hi[1]) {
+        x.y = -x.y + 2.0f * P.wall_hi[1] - off;
+        v.y = -v.y * e;
+    }
+    if (x.x < P.wall_lo[0]) {
+        x.x = -x.x + 2.0f * P.wall_lo[0] + off;
+        v.x = -v.x * e;
+    }
+    if (x.x > P.wall_hi[0]) {
+        x.x = -x.x + 2.0f * P.wall_hi[0] - off;
+        v.x = -v.x * e;
+    }
+    if (x.z < P.wall_lo[2]) {
+        x.z = -x.z + 2.0f * P.wall_lo[2] + off;
+        v.z = -v.z * e;
+    }
+    if (x.z > P.wall_hi[2]) {
+        x.z = -x.z + 2.0f * P.wall_hi[2] - off;
+        v.z = -v.z * e;
+    }
+}
+
+template <bool WRITE_FORCE, bool NEXT_KEYS>
+__global__ __launch_bounds__(kBlock) void k_force_integrate(
+    const float4 *__restrict__ posB, const float4 *__restrict__ velB,
+    const uint32_t *__restrict__ cellcB, const uint32_t *__restrict__ cstart,
+    const float *__restrict__ rhoB, const float2 *__restrict__ pinvB, float4 *__restrict__ posA,
+    float4 *__restrict__ velA, uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA,
+    uint32_t *__restrict__ counts, float4 *__restrict__ force_out, uint32_t n, StepParams P)
+{
+    uint32_t s = particle_index();
+    if (s >= n)
+        return;
+    float4 pi = posB[s];
+    float4 vi = velB[s];
+    const uint32_t cc = cellcB[s];
+    const bool slow = (cc & kSlowBit) != 0u;
+    const RefCell ci = ref_cell(pi, P);
+    const float rho_i = rhoB[s];
+    const float p_i = pinvB[s].x;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for_each_candidate(posB, cstart, s, cc, P, [&](uint32_t j, float4 q) {
+        float dx, dy, dz;
+        float d2 = dist2_ref(pi, q, dx, dy, dz);
+        if (d2 < P.h2) {
+            int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
+            const float2 pj = pinvB[j];
+            const float4 vj = velB[j];
+            const float rinv = __builtin_amdgcn_rsqf(d2);
+            const float r = d2 * rinv;
+            const float hr = P.h - r;
+            float cp = P.neg_m_spg * (p_i + pj.x) * (0.5f * pj.y) * (hr * hr) * rinv;
+            float cv = P.mu_m_lap * hr * pj.y;
+            cp *= (float)m;
+            cv *= (float)m;
+            fx = fmaf(cp, dx, fx);
+            fy = fmaf(cp, dy, fy);
+            fz = fmaf(cp, dz, fz);
+            fx = fmaf(cv, vj.x - vi.x, fx);
+            fy = fmaf(cv, vj.y - vi.y, fy);
+            fz = fmaf(cv, vj.z - vi.z, fz);
+        }
+    });
+    if (WRITE_FORCE)
+        force_out[s] = make_float4(fx, fy, fz, 0.0f);
+    integrate_and_walls(pi, vi, fx, fy, fz, rho_i, P);
+    if (NEXT_KEYS) {
+        Classified c = classify(pi.x, pi.y, pi.z, P);
+        pi.w = __uint_as_float(c.h16);
+        cellcA[s] = c.cellc;
+        rankA[s] = atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
+    } else {
+        pi.w = 0.0f;
+    }
+    posA[s] = pi;
+    velA[s] = vi;
+}
+
+void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
+                            uint32_t n, const StepParams &P, float4 *force_out, bool next_keys)
+{
+    dim3 g(blocks_for(n)), b(kBlock);
+#define SPH_LAUNCH_FI(WF, NK)                                                                   \
+    hipLaunchKernelGGL((k_force_integrate<WF, NK>), g, b, 0, st, a.posB, a.velB, a.cellcB,      \
+                       c.cstart, a.rhoB, a.pinvB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,  \
+                       force_out, n, P)
+    if (force_out && next_keys)
+        SPH_LAUNCH_FI(true, true);
+    else if (force_out)
+        SPH_LAUNCH_FI(true, false);
+    else if (next_keys)
+        SPH_LAUNCH_FI(false, true);
+    else
+        SPH_LAUNCH_FI(false, false);
+#undef SPH_LAUNCH_FI
+}
+
+// ------------------------------------------------------- drop-in output
+// glm::translate(x) * glm::scale(vec3(s)) with glm's column arithmetic
+// (src/sph.cpp:179-180): T[3] = I0 x + I1 y + I2 z + I3, then
+// M[c] = T0 S[c][0] + T1 S[c][1] + T2 S[c][2] + T3 S[c][3].
+__device__ inline void write_transform(float *__restrict__ xf, float x, float y, float z, float sc)
+{
+    const float pos[3] = {x, y, z};
+    float T3[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float a0 = (r == 0) ? 1.0f : 0.0f, a1 = (r == 1) ? 1.0f : 0.0f, a2 = (r == 2) ? 1.0f : 0.0f;
+        float a3 = (r == 3) ? 1.0f : 0.0f;
+        T3[r] = ((a0 * pos[0] + a1 * pos[1]) + a2 * pos[2]) + a3;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float s0 = (c == 0) ? sc : 0.0f, s1 = (c == 1) ? sc : 0.0f;
+        const float s2 = (c == 2) ? sc : 0.0f, s3 = (c == 3) ? 1.0f : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float t0 = (r == 0) ? 1.0f : 0.0f, t1 = (r == 1) ? 1.0f : 0.0f, t2 = (r == 2) ? 1.0f : 0.0f;
+            xf[c * 4 + r] = ((t0 * s0 + t1 * s1) + t2 * s2) + T3[r] * s3;
+        }
+    }
+}
+
+__global__ void k_h16_rank(const float4 *__restrict__ posB, uint32_t *__restrict__ counts16,
+                           uint32_t *__restrict__ rank16, uint32_t n)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    uint32_t h = __float_as_uint(posB[s].w) & 0xFFFFu;
+    rank16[s] = atomicAdd(&counts16[h], 1u);
+}
+
+void launch_h16_rank(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n)
+{
+    // rankA is free once the force pass ran without next keys
+    hipLaunchKernelGGL(k_h16_rank, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.posB, c16.counts,
+                       a.rankA, n);
+}
+
+__global__ void k_h16_scatter(const float4 *__restrict__ posB, const uint32_t *__restrict__ rank16,
+                              const uint32_t *__restrict__ cstart16, uint32_t *__restrict__ perm16,
+                              uint32_t n)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    uint32_t h = __float_as_uint(posB[s].w) & 0xFFFFu;
+    perm16[cstart16[h] + rank16[s]] = s;
+}
+
+void launch_h16_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n,
+                        uint32_t *perm16)
+{
+    hipLaunchKernelGGL(k_h16_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.posB, a.rankA,
+                       c16.cstart, perm16, n);
+}
+
+struct AosOut {
+    float v[15];
+};
+
+// Output slot o: the particle perm16[o] lands at its bucket start plus the
+// number of bucket members with a smaller id (ties of std::sort resolved by
+// input order).
+__global__ void k_pack_out(const uint32_t *__restrict__ perm16, const uint32_t *__restrict__ cstart16,
+                           const float4 *__restrict__ posB, const float4 *__restrict__ velB,
+                           const float *__restrict__ rhoB, const float2 *__restrict__ pinvB,
+                           const float4 *__restrict__ force, const float4 *__restrict__ posA,
+                           const float4 *__restrict__ velA, const float *__restrict__ aos_in,
+                           float *__restrict__ aos_out, float *__restrict__ xf_out, uint32_t n,
+                           float sc)
+{
+    uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n)
+        return;
+    const uint32_t s = perm16[o];
+    const uint32_t h = __float_as_uint(posB[s].w) & 0xFFFFu;
+    const uint32_t id = __float_as_uint(velB[s].w);
+    const uint32_t b = cstart16[h], e = cstart16[h + 1];
+    uint32_t d;
+    if (e - b <= kStableCellMax) {
+        uint32_t r = 0;
+        for (uint32_t t = b; t < e; ++t)
+            r += (__float_as_uint(velB[perm16[t]].w) < id) ? 1u : 0u;
+        d = b + r;
+    } else {
+        d = o;
+    }
+    const float4 x = posA[s], v = velA[s], f = force[s];
+    const float *src = aos_in + (size_t)id * 15;
+    float *dst = aos_out + (size_t)d * 15;
+    dst[0] = x.x;
+    dst[1] = x.y;
+    dst[2] = x.z;
+    dst[3] = v.x;
+    dst[4] = v.y;
+    dst[5] = v.z;
+    dst[6] = src[6];
+    dst[7] = src[7];
+    dst[8] = src[8];
+    dst[9] = f.x;
+    dst[10] = f.y;
+    dst[11] = f.z;
+    dst[12] = rhoB[s];
+    dst[13] = pinvB[s].x;
+    dst[14] = __uint_as_float(h);  // uint16 hash + zero padding (little endian)
+    if (xf_out)
+        write_transform(xf_out + (size_t)d * 16, x.x, x.y, x.z, sc);
+}
+
+void launch_pack_out(hipStream_t st, const ParticleArrays &a, const CellArrays &c16,
+                     const uint32_t *perm16, const float4 *force, const void *aos_in, uint32_t n,
+                     const StepParams &P, void *aos_out, float *xf_out)
+{
+    hipLaunchKernelGGL(k_pack_out, dim3(blocks_for(n)), dim3(kBlock), 0, st, perm16, c16.cstart,
+                       a.posB, a.velB, a.rhoB, a.pinvB, force, a.posA, a.velA,
+                       (const float *)aos_in, (float *)aos_out, xf_out, n, P.xf_scale);
+}
+
+// Resident download: slot s of the A arrays (device order).  Bucket, density
+// and pressure belong to the last step (B slot s is the same particle).
+__global__ void k_pack_state(const float4 *__restrict__ posA, const float4 *__restrict__ velA,
+                             const float4 *__restrict__ posB, const float *__restrict__ rhoB,
+                             const float2 *__restrict__ pinvB, const float4 *__restrict__ force,
+                             float *__restrict__ aos_out, float *__restrict__ xf_out, uint32_t n,
+                             float sc)
+{
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n)
+        return;
+    const float4 x = posA[s], v = velA[s];
+    const float4 f = force ? force[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float *dst = aos_out + (size_t)s * 15;
+    dst[0] = x.x;
+    dst[1] = x.y;
+    dst[2] = x.z;
+    dst[3] = v.x;
+    dst[4] = v.y;
+    dst[5] = v.z;
+    dst[6] = 0.0f;
+    dst[7] = 0.0f;
+    dst[8] = 0.0f;
+    dst[9] = f.x;
+    dst[10] = f.y;
+    dst[11] = f.z;
+    dst[12] = rhoB[s];
+    dst[13] = pinvB[s].x;
+    dst[14] = __uint_as_float(__float_as_uint(posB[s].w) & 0xFFFFu);
+    if (xf_out)
+        write_transform(xf_out + (size_t)s * 16, x.x, x.y, x.z, sc);
+}
+
+void launch_pack_state(hipStream_t st, const ParticleArrays &a, const float4 *force, uint32_t n,
+                       const StepParams &P, void *aos_out, float *xf_out)
+{
+    hipLaunchKernelGGL(k_pack_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.posA, a.velA,
+                       a.posB, a.rhoB, a.pinvB, force, (float *)aos_out, xf_out, n, P.xf_scale);
+}
+
+// ------------------------------------------------------------------ copy
+
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride)
+        dst[i] = src[i];
+}
+
+void launch_copy(hipStream_t st, const uint4 *src, uint4 *dst, size_t n16)
+{
+    size_t nb = (n16 + kBlock - 1) / kBlock;
+    if (nb > 65536)
+        nb = 65536;
+    hipLaunchKernelGGL(k_copy, dim3((uint32_t)nb), dim3(kBlock), 0, st, src, dst, n16);
+}
+
+}  // namespace sph

@@ -1,0 +1,79 @@
+// sph_runtime.h -- internal host-side pieces shared by the runtime files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sph_hip.h"
+#include "sph_kernels.h"
+
+namespace sph {
+
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+const char *last_error();
+int ensure_device();
+
+void settings_init(sph_settings_v1 *s, float mass, float restDensity, float gasConst,
+                   float viscosity, float h, float g, float tension);
+void box_reference_cpu(sph_box_v1 *b);
+void box_reference_gpu(sph_box_v1 *b);
+int check_settings(const sph_settings_v1 *s);
+StepParams make_params(const sph_settings_v1 &s, const sph_box_v1 &box, float dt);
+int make_grid(StepParams &P, const double lo[3], const double hi[3], double cell_scale,
+              uint64_t max_cells);
+
+// Growable device allocation (hipMalloc on growth only).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want);
+    void release();
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+};
+
+// All device arrays of one particle set (see sph_kernels.h for the layout).
+struct Workspace {
+    DevBuf posA, velA, posB, velB, cellcA, rankA, cellcB, perm, rhoB, pinvB;
+    DevBuf counts, cstart, partials;
+    DevBuf force;
+    uint32_t ncells_padded = 0;
+    bool counts_dirty = true;
+    int reserve_particles(uint32_t n);
+    int reserve_cells(uint64_t ncells);
+    ParticleArrays particles() const;
+    CellArrays cells() const;
+    void release_all();
+};
+
+// HIP events recorded between kernels on one stream.
+struct StepTimer {
+    std::vector<hipEvent_t> events;
+    std::vector<const char *> names;  // names[k] labels events[k-1] -> events[k]
+    void mark(hipStream_t st, const char *name)
+    {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, st);
+        events.push_back(e);
+        names.push_back(name);
+    }
+    void clear()
+    {
+        for (hipEvent_t e : events)
+            (void)hipEventDestroy(e);
+        events.clear();
+        names.clear();
+    }
+    ~StepTimer() { clear(); }
+};
+
+void enqueue_step(hipStream_t st, Workspace &w, uint32_t n, const StepParams &P, float4 *force_out,
+                  bool next_keys, StepTimer *timer);
+
+}  // namespace sph

@@ -1,0 +1,293 @@
+"""ctypes binding of libsph_hip.so (the C ABI declared in include/sph_hip.h).
+
+Mirrors the reference interface of lijenicol/SPH-Fluid-Simulator for the SPH
+step:  ``settings(...)`` is ``SPHSettings(mass, restDensity, gasConst,
+viscosity, h, g, tension)`` (src/SPHSystem.h:15-17), ``update_particles`` is
+``updateParticles(particles, transforms, n, settings, dt, onGPU=true)``
+(src/sph.h:19-22) with the CPU path's semantics, and ``Context`` is the
+resident (state kept in HBM) form of ``SPHSystem::update``.
+
+The library must be built (``make`` in the package directory); loading fails
+loudly otherwise.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PACKAGE_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PACKAGE_DIR, "lib", "libsph_hip.so")
+
+SPH_OK = 0
+SPH_E_INVALID = -1
+SPH_E_NODEVICE = -2
+SPH_E_NOMEM = -3
+SPH_E_HIP = -4
+SPH_E_LIMIT = -5
+SPH_E_STATE = -6
+SPH_SCENE_KEEP_FORCE = 1
+
+# struct Particle (src/Particle.h:4-10): 60 bytes.
+PARTICLE_DTYPE = np.dtype(
+    {
+        "names": ["position", "velocity", "acceleration", "force", "density", "pressure", "hash", "_pad"],
+        "formats": [("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 3), "<f4", "<f4", "<u2", "<u2"],
+        "offsets": [0, 12, 24, 36, 48, 52, 56, 58],
+        "itemsize": 60,
+    }
+)
+
+
+class Settings(ctypes.Structure):
+    """struct SPHSettings (src/SPHSystem.h:13-22), 116 bytes."""
+
+    _fields_ = [("sphereScale", c_float * 16)] + [
+        (name, c_float)
+        for name in (
+            "poly6", "spikyGrad", "spikyLap", "gasConstant", "mass", "h2", "selfDens",
+            "restDensity", "viscosity", "h", "g", "tension", "massPoly6Product",
+        )
+    ]
+
+
+class Box(ctypes.Structure):
+    _fields_ = [("lo", c_float * 3), ("hi", c_float * 3), ("restitution", c_float), ("offset", c_float)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", c_uint32),
+        ("dt", c_float),
+        ("settings", Settings),
+        ("box", Box),
+        ("cell_scale", c_float),
+        ("flags", c_uint32),
+    ]
+
+
+class SphError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"sph_hip error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """Loads the HIP library (once) and declares the C signatures."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is not built; run `make` in {PACKAGE_DIR}")
+    lib = ctypes.CDLL(p)
+    sig = {
+        "sph_abi_version": (c_int, []),
+        "sph_hip_last_error": (c_char_p, []),
+        "sph_settings_init": (c_int, [POINTER(Settings)] + [c_float] * 7),
+        "sph_box_reference_cpu": (c_int, [POINTER(Box)]),
+        "sph_box_reference_gpu": (c_int, [POINTER(Box)]),
+        "sph_get_hash": (c_uint32, [ctypes.c_int32] * 3),
+        "sph_classify_host": (c_int, [POINTER(c_float), c_float, POINTER(ctypes.c_int32)]),
+        "sph_hip_update_particles": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(Settings), c_float]),
+        "sph_hip_update_particles_box": (
+            c_int, [c_void_p, c_void_p, c_size_t, POINTER(Settings), c_float, POINTER(Box)]),
+        "sph_hip_neighbors": (
+            c_int, [c_void_p, c_size_t, POINTER(Settings), c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
+        "sph_hip_create": (c_int, [POINTER(SceneDesc), POINTER(c_void_p)]),
+        "sph_hip_destroy": (c_int, [c_void_p]),
+        "sph_hip_upload": (c_int, [c_void_p, c_void_p, c_void_p]),
+        "sph_hip_step": (c_int, [c_void_p, c_int]),
+        "sph_hip_sync": (c_int, [c_void_p]),
+        "sph_hip_download": (c_int, [c_void_p] * 6),
+        "sph_hip_download_particles": (c_int, [c_void_p, c_void_p, c_void_p]),
+        "sph_hip_set_graph": (c_int, [c_void_p, c_int]),
+        "sph_hip_count": (c_uint32, [c_void_p]),
+        "sph_hip_profile": (
+            c_int, [c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_int), c_char_p, c_size_t]),
+        "sph_hip_copy_bench": (c_int, [c_size_t, c_int, POINTER(c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != SPH_OK:
+        msg = load().sph_hip_last_error()
+        raise SphError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a: np.ndarray | None) -> c_void_p | None:
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def settings(mass=0.02, rest_density=1000.0, gas_const=1.0, viscosity=1.04, h=0.15, g=-9.8,
+             tension=0.2) -> Settings:
+    """SPHSettings(mass, restDensity, gasConst, viscosity, h, g, tension)."""
+    s = Settings()
+    _check(load().sph_settings_init(ctypes.byref(s), mass, rest_density, gas_const, viscosity, h, g, tension))
+    return s
+
+
+def box_reference_cpu() -> Box:
+    b = Box()
+    _check(load().sph_box_reference_cpu(ctypes.byref(b)))
+    return b
+
+
+def box_reference_gpu() -> Box:
+    b = Box()
+    _check(load().sph_box_reference_gpu(ctypes.byref(b)))
+    return b
+
+
+def box(lo, hi, restitution=0.5, offset=1e-4) -> Box:
+    b = Box()
+    for k in range(3):
+        b.lo[k] = lo[k]
+        b.hi[k] = hi[k]
+    b.restitution = restitution
+    b.offset = offset
+    return b
+
+
+def get_hash(cx: int, cy: int, cz: int) -> int:
+    return int(load().sph_get_hash(cx, cy, cz))
+
+
+def classify_host(pos, h: float) -> tuple[int, bool, tuple[int, int, int]]:
+    p = (c_float * 3)(*[float(v) for v in pos])
+    out = (ctypes.c_int32 * 5)()
+    _check(load().sph_classify_host(p, h, out))
+    return int(out[0]), bool(out[1]), (int(out[2]), int(out[3]), int(out[4]))
+
+
+def make_particles(pos: np.ndarray, vel: np.ndarray | None = None) -> np.ndarray:
+    """Reference Particle records from n x 3 positions (and velocities)."""
+    pos = np.asarray(pos, dtype=np.float32)
+    p = np.zeros(len(pos), dtype=PARTICLE_DTYPE)
+    p["position"] = pos
+    if vel is not None:
+        p["velocity"] = np.asarray(vel, dtype=np.float32)
+    return p
+
+
+def update_particles(particles: np.ndarray, s: Settings, dt: float, box_: Box | None = None,
+                     transforms: bool = True):
+    """One drop-in step, in place on a copy.  Returns (particles, transforms)."""
+    p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE).copy()
+    n = len(p)
+    xf = np.zeros((n, 16), dtype=np.float32) if transforms else None
+    lib = load()
+    if box_ is None:
+        rc = lib.sph_hip_update_particles(_ptr(p), _ptr(xf), n, ctypes.byref(s), dt)
+    else:
+        rc = lib.sph_hip_update_particles_box(_ptr(p), _ptr(xf), n, ctypes.byref(s), dt, ctypes.byref(box_))
+    _check(rc)
+    return p, xf
+
+
+def neighbors(particles: np.ndarray, s: Settings):
+    """Neighbour multisets (CSR over input indices, multiplicity-repeated)."""
+    p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+    n = len(p)
+    off = np.zeros(n + 1, dtype=np.uint64)
+    total = c_size_t(0)
+    lib = load()
+    _check(lib.sph_hip_neighbors(_ptr(p), n, ctypes.byref(s), _ptr(off), None, 0, ctypes.byref(total)))
+    idx = np.zeros(max(int(total.value), 1), dtype=np.uint32)
+    _check(lib.sph_hip_neighbors(_ptr(p), n, ctypes.byref(s), _ptr(off), _ptr(idx), len(idx),
+                                 ctypes.byref(total)))
+    return off, idx[: int(total.value)]
+
+
+class Context:
+    """Resident single scene: SPHSystem::update without per-step PCIe."""
+
+    def __init__(self, n: int, s: Settings, box_: Box, dt: float, cell_scale: float = 1.0,
+                 keep_force: bool = False):
+        self._lib = load()
+        d = SceneDesc()
+        d.n = n
+        d.dt = dt
+        d.settings = s
+        d.box = box_
+        d.cell_scale = cell_scale
+        d.flags = SPH_SCENE_KEEP_FORCE if keep_force else 0
+        h = c_void_p()
+        _check(self._lib.sph_hip_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.n = n
+
+    def close(self):
+        if self._h:
+            self._lib.sph_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, pos: np.ndarray, vel: np.ndarray | None = None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(self.n, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(self.n, 3)
+        _check(self._lib.sph_hip_upload(self._h, _ptr(pos), _ptr(vel)))
+
+    def step(self, nsteps: int = 1):
+        _check(self._lib.sph_hip_step(self._h, nsteps))
+
+    def sync(self):
+        _check(self._lib.sph_hip_sync(self._h))
+
+    def set_graph(self, enable: bool):
+        _check(self._lib.sph_hip_set_graph(self._h, 1 if enable else 0))
+
+    def download(self):
+        n = self.n
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        rho = np.zeros(n, np.float32)
+        prs = np.zeros(n, np.float32)
+        ids = np.zeros(n, np.uint32)
+        _check(self._lib.sph_hip_download(self._h, _ptr(pos), _ptr(vel), _ptr(rho), _ptr(prs), _ptr(ids)))
+        return {"pos": pos, "vel": vel, "rho": rho, "pressure": prs, "ids": ids}
+
+    def download_particles(self, transforms: bool = True):
+        p = np.zeros(self.n, dtype=PARTICLE_DTYPE)
+        xf = np.zeros((self.n, 16), np.float32) if transforms else None
+        _check(self._lib.sph_hip_download_particles(self._h, _ptr(p), _ptr(xf)))
+        return p, xf
+
+    def profile(self, nsteps: int = 10) -> dict:
+        ms = (c_double * 32)()
+        nk = c_int(0)
+        names = ctypes.create_string_buffer(1024)
+        _check(self._lib.sph_hip_profile(self._h, nsteps, ms, 32, ctypes.byref(nk), names, 1024))
+        labels = names.raw.split(b"\0")[: nk.value]
+        return {labels[k].decode(): float(ms[k]) for k in range(nk.value)}
+
+
+def copy_bench(nbytes: int, iters: int = 20) -> float:
+    """Mean milliseconds of one device copy of nbytes (read nbytes + write nbytes)."""
+    ms = c_double(0.0)
+    _check(load().sph_hip_copy_bench(nbytes, iters, ctypes.byref(ms)))
+    return ms.value

@@ -1,0 +1,200 @@
+"""CPU tests of libsph_hip.so: it loads, exports every symbol the header
+declares, mirrors the reference structs and constants, and its per-particle
+classification (the rule that lets the dense-grid kernels reproduce the
+reference's 16-bit bucket multiplicities) holds against brute force.  No
+device work is done here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import brute
+
+F32 = np.float32
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "sph_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sph_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(sph):
+    names = declared_functions()
+    assert len(names) >= 20
+    lib = ctypes.CDLL(sph.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_abi_version_and_struct_sizes(sph):
+    assert sph.load().sph_abi_version() == 1
+    assert sph.PARTICLE_DTYPE.itemsize == 60
+    assert ctypes.sizeof(sph.Settings) == 116
+    assert ctypes.sizeof(sph.Box) == 32
+
+
+@pytest.mark.parametrize("args", [(0.02, 1000.0, 1.0, 1.04, 0.15, -9.8, 0.2),
+                                  (7.4074074e-4, 1000.0, 1.0, 1.04, 0.05, -9.8, 0.2),
+                                  (1.6e-4, 1000.0, 1.0, 1.04, 0.03, -9.8, 0.2),
+                                  (0.5, 800.0, 3.0, 0.5, 0.7, -1.0, 1.0)])
+def test_settings_bitwise_equal_to_oracle(sph, orc32, args):
+    a = sph.settings(*args)
+    b = orc32.settings(*args)
+    assert bytes(a) == bytes(b)
+
+
+def test_box_presets(sph, orc32):
+    a = sph.box_reference_cpu()
+    b = orc32.box_reference_cpu()
+    assert bytes(a) == bytes(b)
+    g = sph.box_reference_gpu()
+    assert list(g.lo) == [-10, 0, -10] and g.hi[0] == 10 and g.restitution == 1.0
+    assert g.offset == F32(1e-5)
+
+
+def test_hash_matches_oracle(sph, orc32):
+    rng = np.random.default_rng(0)
+    for c in rng.integers(-100000, 100000, size=(200, 3)):
+        assert sph.get_hash(*map(int, c)) == orc32.get_hash(*map(int, c))
+
+
+def test_classify_matches_reference_cell_and_bucket(sph, orc32):
+    rng = np.random.default_rng(1)
+    h = 0.15
+    pos = rng.uniform(-3, 3, size=(500, 3)).astype(F32)
+    for p in pos:
+        h16, _, cell = sph.classify_host(p, h)
+        assert cell == orc32.get_cell(tuple(map(float, p)), h)
+        assert h16 == orc32.get_hash(*cell) & 0xFFFF
+
+
+def test_slow_flag_covers_every_bucket_collision(sph):
+    """buckets_collide (GF(2) rank test) agrees with a brute-force check of
+    the 27 buckets; particles in colliding cells are always slow."""
+    h = 1.0
+    rng = np.random.default_rng(2)
+    cells = np.concatenate([rng.integers(-300, 300, size=(4000, 3)),
+                            np.array([[x, 0, 0] for x in range(-20, 20)] + [[0, y, 0] for y in range(-20, 20)] +
+                                     [[0, 0, z] for z in range(-20, 20)] + [[5, 0, 7], [0, 0, 0]])])
+    hits = 0
+    for c in cells:
+        # centre of the reference cell (cell 0 spans (-h, h))
+        centre = [(ci + 0.5 * np.sign(ci)) * h if ci != 0 else 0.0 for ci in c]
+        _, slow, cell = sph.classify_host(centre, h)
+        assert cell == tuple(int(v) for v in c)
+        collide = brute.buckets_collide_brute(c)
+        hits += collide
+        if collide:
+            assert slow, c
+        else:
+            assert not slow, c  # the centre is far from boundaries
+    assert hits > 50
+
+
+def test_near_boundary_particles_are_slow(sph):
+    h = F32(0.05)
+    for k in (1, 2, 7, -1, -3, 100):
+        b = F32(k) * h
+        for eps in (0.0, 1e-7, 1e-5, 5e-5):  # all within 2^-9 h of the boundary
+            for s in (-1, 1):
+                x = F32(b + F32(s * eps))
+                _, slow, _ = sph.classify_host((x, F32(0.61) * h + F32(3.3), F32(5.43)), float(h))
+                assert slow, (k, eps, s)
+
+
+def _fast_path_holds(sph, pos, h):
+    """For every particle the classifier calls fast, every brute-force
+    neighbour has multiplicity exactly 1."""
+    rows = brute.neighbors_brute(pos, h)
+    n_fast = 0
+    for i, row in enumerate(rows):
+        _, slow, _ = sph.classify_host(pos[i], h)
+        if not slow:
+            n_fast += 1
+            assert len(row) == len(np.unique(row)), i
+            # and no neighbour is missing: brute force counts every j with
+            # fl(d2) < h2 whose bucket is among the 27 (m >= 1)
+            d2 = brute.dist2_rows(pos, i)
+            within = np.nonzero(d2 < F32(h) * F32(h))[0]
+            within = within[within != i]
+            assert np.array_equal(np.sort(within), row), i
+    return n_fast
+
+
+def test_fast_path_theorem_random_cloud(sph):
+    h = 0.15
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-3 * h, 3 * h, size=(1500, 3)).astype(F32)
+    assert _fast_path_holds(sph, pos, h) > 500
+
+
+def test_fast_path_theorem_boundary_adversarial(sph):
+    """Pairs straddling reference cell boundaries at distance just below h."""
+    h = F32(0.05)
+    rng = np.random.default_rng(4)
+    pts = []
+    for _ in range(300):
+        k = int(rng.integers(-40, 40))
+        b = F32(k) * h
+        y, z = rng.uniform(-1, 1, size=2).astype(F32)
+        dx = F32(rng.uniform(0.99, 1.0)) * h
+        x0 = F32(b - F32(rng.uniform(0, 2e-6)))
+        pts.append((x0, y, z))
+        pts.append((F32(x0 + dx), y, z))
+    pos = np.array(pts, dtype=F32)
+    _fast_path_holds(sph, pos, float(h))
+
+
+def test_grid_cells_of_neighbours_are_adjacent():
+    """Completeness of the 3x3x3 grid stencil: with H = h (1 + 2^-10) and
+    double-precision grid coordinates, any pair with fl(d^2) < fl(h*h) lies
+    in adjacent grid cells, including pairs at distance h - ulp across a
+    grid boundary."""
+    h = F32(0.03)
+    H = float(h) * (1.0 + 2.0 ** -10)
+    origin = np.array([-2.0 - H, -1.0 - H, -1.0 - H])
+    inv = 1.0 / H
+    dims = (140, 70, 70)
+    rng = np.random.default_rng(5)
+    pairs = []
+    for _ in range(20000):
+        g = rng.integers(1, 60)
+        a = F32(origin[0] + g * H) - F32(rng.uniform(0, 1e-6))
+        d = F32(rng.uniform(0.999, 1.0)) * h
+        y, z = rng.uniform(-0.9, 0.9, size=2).astype(F32)
+        pairs.append(((a, y, z), (F32(a + d), y, z)))
+    p = np.array([q for pr in pairs for q in pr], dtype=F32)
+    gc = brute.grid_coords(p, origin, inv, dims)
+    h2 = h * h
+    checked = 0
+    for k in range(len(pairs)):
+        i, j = 2 * k, 2 * k + 1
+        dv = p[j] - p[i]
+        d2 = (dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2]
+        if d2 < h2:
+            checked += 1
+            assert np.all(np.abs(gc[i] - gc[j]) <= 1)
+    assert checked > 1000
+
+
+def test_invalid_arguments_fail_without_touching_a_device(sph):
+    lib = sph.load()
+    s = sph.settings()
+    assert lib.sph_hip_update_particles(None, None, 5, ctypes.byref(s), 0.003) == sph.SPH_E_INVALID
+    assert b"NULL" in lib.sph_hip_last_error()
+    assert lib.sph_hip_update_particles(None, None, 0, ctypes.byref(s), 0.003) == sph.SPH_OK
+    bad = sph.settings(h=0.15)
+    bad.h = 0.0
+    p = sph.make_particles(np.zeros((3, 3), F32))
+    assert lib.sph_hip_update_particles(p.ctypes.data_as(ctypes.c_void_p), None, 3, ctypes.byref(bad),
+                                        0.003) == sph.SPH_E_INVALID
+    d = sph.SceneDesc()
+    d.n = 0
+    h = ctypes.c_void_p()
+    assert lib.sph_hip_create(ctypes.byref(d), ctypes.byref(h)) == sph.SPH_E_INVALID
+    assert lib.sph_hip_step(None, 1) == sph.SPH_E_INVALID

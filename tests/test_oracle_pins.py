@@ -1,0 +1,345 @@
+"""Pins the oracle (CPU restatement of the reference step) against closed
+forms, brute force and invariants.  The reference ships no tests or golden
+vectors and cannot be compiled here (glm/GLEW/GLUT are absent), so these
+checks are what fixes the oracle; each is chosen so a plausible slip (wrong
+exponent, sign, operand, dropped multiplicity) fails it."""
+import math
+
+import numpy as np
+import pytest
+
+import brute
+
+F32 = np.float32
+
+
+# ---------------------------------------------------------------- settings
+
+@pytest.mark.parametrize("mass,h", [(0.02, 0.15), (0.02 * (0.05 / 0.15) ** 3, 0.05), (1.6e-4, 0.03)])
+def test_settings_constants_closed_form(orc32, mass, h):
+    s = orc32.settings(mass, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    hf, mf = float(F32(h)), float(F32(mass))
+    ref = brute.reference_constants(mf, hf)
+    # textbook forms (true pi) to 1e-6: catches a wrong power or factor
+    assert s.poly6 == pytest.approx(315.0 / (64.0 * math.pi * hf ** 9), rel=1e-6)
+    assert s.spikyGrad == pytest.approx(-45.0 / (math.pi * hf ** 6), rel=1e-6)
+    assert s.spikyLap == pytest.approx(45.0 / (math.pi * hf ** 6), rel=1e-6)
+    assert s.selfDens == pytest.approx(mf * 315.0 / (64.0 * math.pi * hf ** 3), rel=1e-6)
+    # the reference's promotion order (double pow, float PI) to the last bit
+    assert F32(s.poly6) == F32(ref["poly6"])
+    assert F32(s.spikyGrad) == F32(ref["spikyGrad"])
+    assert F32(s.spikyLap) == F32(ref["spikyLap"])
+    assert s.h2 == F32(hf) * F32(hf)
+    assert s.massPoly6Product == F32(mf) * F32(s.poly6)
+    assert list(s.sphereScale) == [F32(hf) / 2, 0, 0, 0, 0, F32(hf) / 2, 0, 0, 0, 0, F32(hf) / 2, 0, 0, 0, 0, 1]
+
+
+def test_settings_published_probe_values(orc32):
+    # independent probe values for h = 0.15, m = 0.02
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, 0.15, -9.8, 0.2)
+    assert s.poly6 == pytest.approx(40752964.0, rel=1e-7)
+    assert s.spikyGrad == pytest.approx(-1257520.25, rel=1e-7)
+    assert s.selfDens == pytest.approx(9.28403664, rel=1e-6)
+    assert s.massPoly6Product == pytest.approx(815059.25, rel=1e-7)
+
+
+# ------------------------------------------------------------- hash / cell
+
+def test_hash_closed_form(orc32):
+    for c in [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (-1, 2, -3), (100, -57, 66), (-70000, 3, 123456)]:
+        cx, cy, cz = (v & 0xFFFFFFFF for v in c)
+        expect = (((cx * 73856093) ^ (cy * 19349663) ^ (cz * 83492791)) & 0xFFFFFFFF) % 262144
+        assert orc32.get_hash(*c) == expect
+    assert orc32.get_hash(1, 0, 0) == 73856093 % 262144
+
+
+def test_hash_axis_double_count_identity(orc32):
+    # odd multipliers: -a ^ b == a ^ -b, so (x,-1,1) and (x,1,-1) share a bucket
+    for x in (-5, 0, 3, 40):
+        assert orc32.get_hash(x, -1, 1) == orc32.get_hash(x, 1, -1)
+        assert orc32.get_hash(-1, x, 1) == orc32.get_hash(1, x, -1)
+    assert brute.buckets_collide_brute((7, 0, 0))
+
+
+def test_get_cell_truncates_toward_zero(orc32):
+    h = 0.15
+    assert orc32.get_cell((-0.5 * h, 0.5 * h, 1.2 * h), h) == (0, 0, 1)
+    assert orc32.get_cell((-1.2 * h, -2.7 * h, 2.0 * h), h) == (-1, -2, 2)
+    assert orc32.get_cell((float("nan"), 1e30, -1e30), h) == (-2 ** 31, -2 ** 31, -2 ** 31)
+
+
+# --------------------------------------------------------------- neighbours
+
+def _oracle_rows_by_id(orc, pos, h):
+    s = orc.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    p = orc.particles(pos, ids=np.arange(len(pos)))
+    table = orc.prepare(p, s, nthreads=2)
+    off, idx = orc.neighbors(p, table, s)
+    orc.free_table(table)
+    ids = p["acceleration"][:, 0].astype(np.int64)
+    rows = [None] * len(pos)
+    for si in range(len(p)):
+        rows[ids[si]] = np.sort(ids[idx[off[si]:off[si + 1]]])
+    return rows
+
+
+def _axis_cloud(n, h, seed):
+    rng = np.random.default_rng(seed)
+    # straddles x = 0, y = 0, z = 0 so the double-counted columns are hit
+    pos = rng.uniform(-2.5 * h, 2.5 * h, size=(n, 3))
+    return pos.astype(F32)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_neighbor_multisets_match_brute_force(orc32, seed):
+    h = 0.15
+    pos = _axis_cloud(1200, h, seed)
+    rows_o = _oracle_rows_by_id(orc32, pos, h)
+    rows_b = brute.neighbors_brute(pos, h)
+    doubled = 0
+    for i in range(len(pos)):
+        assert np.array_equal(rows_o[i], rows_b[i]), f"particle {i}"
+        doubled += int(len(rows_b[i]) != len(np.unique(rows_b[i])))
+    assert doubled > 0  # the multiplicity rule is exercised
+
+
+def test_neighbor_multisets_far_from_axes(orc32):
+    h = 0.05
+    rng = np.random.default_rng(7)
+    pos = (rng.uniform(0.0, 6 * h, size=(1500, 3)) + np.array([1.3, 0.7, 2.1])).astype(F32)
+    rows_o = _oracle_rows_by_id(orc32, pos, h)
+    rows_b = brute.neighbors_brute(pos, h)
+    for i in range(len(pos)):
+        assert np.array_equal(rows_o[i], rows_b[i])
+
+
+# ------------------------------------------------------------------ density
+
+def _density_by_id(orc, pos, s, nthreads=2):
+    p = orc.particles(pos, ids=np.arange(len(pos)))
+    table = orc.prepare(p, s, nthreads)
+    orc.density(p, table, s, nthreads)
+    orc.free_table(table)
+    ids = p["acceleration"][:, 0].astype(np.int64)
+    rho = np.zeros(len(pos))
+    prs = np.zeros(len(pos))
+    rho[ids] = p["density"]
+    prs[ids] = p["pressure"]
+    return rho, prs
+
+
+def test_density_isolated_particle(orc32):
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, 0.15, -9.8, 0.2)
+    pos = np.array([[0.3, 0.4, 0.5], [3.0, 3.0, 3.0]], dtype=F32)
+    rho, prs = _density_by_id(orc32, pos, s)
+    assert rho[0] == F32(s.selfDens)
+    assert prs[0] == F32(F32(s.gasConstant) * (F32(s.selfDens) - F32(s.restDensity)))
+
+
+def test_density_lattice_closed_form(orc32):
+    # unjittered lattice, spacing 0.55 h, far from the axes: interior
+    # neighbours at r = 0.55h (6), 0.55h*sqrt2 (12), 0.55h*sqrt3 (8)
+    h = 0.15
+    a = 0.55 * h
+    n1 = 7
+    g = np.arange(n1) * a
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = (np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + np.array([2.03, 1.51, 3.07])).astype(F32)
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    rho, _ = _density_by_id(orc32, pos, s)
+    centre = (3 * n1 + 3) * n1 + 3
+    q = [1 - 0.55 ** 2, 1 - 2 * 0.55 ** 2, 1 - 3 * 0.55 ** 2]
+    closed = float(s.selfDens) * (1 + 6 * q[0] ** 3 + 12 * q[1] ** 3 + 8 * q[2] ** 3)
+    rows = brute.neighbors_brute(pos, h)
+    assert len(rows[centre]) == 26 and len(np.unique(rows[centre])) == 26
+    assert rho[centre] == pytest.approx(closed, rel=2e-6)
+
+
+def test_density_matches_brute_sum_with_multiplicity(orc32):
+    h = 0.15
+    pos = _axis_cloud(900, h, 11)
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    rho, prs = _density_by_id(orc32, pos, s)
+    rows = brute.neighbors_brute(pos, h)
+    h2 = F32(h) * F32(h)
+    mp6 = float(s.massPoly6Product)
+    for i in range(len(pos)):
+        d2 = brute.dist2_rows(pos, i)[rows[i]].astype(np.float64)
+        expect = float(s.selfDens) + mp6 * np.sum((float(h2) - d2) ** 3)
+        assert rho[i] == pytest.approx(expect, rel=2e-6, abs=1e-6)
+        assert prs[i] == pytest.approx(float(s.gasConstant) * (rho[i] - float(s.restDensity)), rel=1e-6)
+
+
+# ------------------------------------------------------------------- forces
+
+def _step_by_id(orc, pos, vel, s, dt, box, nthreads=2):
+    p = orc.particles(pos, vel, ids=np.arange(len(pos)))
+    xf = orc.update(p, s, dt, box, nthreads=nthreads)
+    ids = p["acceleration"][:, 0].astype(np.int64)
+    out = np.zeros(len(pos), dtype=p.dtype)
+    out[ids] = p
+    xfo = np.zeros_like(xf)
+    xfo[ids] = xf
+    return out, xfo
+
+
+def test_two_particle_force_closed_form(orc32):
+    h, m, mu, k, rho0 = 0.15, 0.02, 1.04, 1.0, 1000.0
+    s = orc32.settings(m, rho0, k, mu, h, -9.8, 0.2)
+    box = orc32.box((-50, -50, -50), (50, 50, 50))
+    r = 0.4 * h
+    pos = np.array([[1.01, 1.02, 1.03], [1.01 + r, 1.02, 1.03]], dtype=F32)
+    vel = np.array([[0.1, -0.2, 0.3], [-0.4, 0.25, 0.05]], dtype=F32)
+    out, _ = _step_by_id(orc32, pos, vel, s, 0.0, box)
+    c = brute.reference_constants(float(F32(m)), float(F32(h)))
+    rr = float(pos[1, 0]) - float(pos[0, 0])
+    rho = c["selfDens"] + m * c["poly6"] * (float(F32(h) * F32(h)) - rr * rr) ** 3
+    p = k * (rho - rho0)
+    # F_0: dir = +x;  pressure = -dir m (p+p) / (2 rho) spikyGrad (h-r)^2
+    fp = -1.0 * m * (2 * p) / (2 * rho) * c["spikyGrad"] * (h - rr) ** 2
+    dv = vel[1].astype(np.float64) - vel[0].astype(np.float64)
+    fv = mu * m * dv / rho * c["spikyLap"] * (h - rr)
+    expect0 = np.array([fp, 0.0, 0.0]) + fv
+    expect1 = np.array([-fp, 0.0, 0.0]) - fv
+    assert out["density"][0] == pytest.approx(rho, rel=1e-6)
+    np.testing.assert_allclose(out["force"][0], expect0, rtol=2e-5, atol=1e-3)
+    np.testing.assert_allclose(out["force"][1], expect1, rtol=2e-5, atol=1e-3)
+    # with equal p and rho the pair force is antisymmetric
+    np.testing.assert_allclose(out["force"][0], -out["force"][1], rtol=1e-6, atol=1e-6)
+    assert fp < 0  # the reference's sign: negative pressure pushes the pair apart
+
+
+def test_isolated_particle_has_zero_force_and_falls(orc32):
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, 0.15, -9.8, 0.2)
+    box = orc32.box((-50, -50, -50), (50, 50, 50))
+    pos = np.array([[0.3, 5.0, 0.2]], dtype=F32)
+    vel = np.zeros((1, 3), dtype=F32)
+    dt = 0.003
+    p = orc32.particles(pos, vel)
+    for _ in range(100):
+        orc32.update(p, s, dt, box, nthreads=1)
+    assert np.all(p["force"] == 0)
+    k = 100
+    assert p["velocity"][0, 1] == pytest.approx(-9.8 * dt * k, rel=1e-5)
+    # symplectic Euler: x_k = x_0 + g dt^2 k (k + 1) / 2
+    assert p["position"][0, 1] == pytest.approx(5.0 - 9.8 * dt * dt * k * (k + 1) / 2, rel=1e-5)
+    assert p["velocity"][0, 0] == 0 and p["velocity"][0, 2] == 0
+
+
+# ---------------------------------------------------------------- integrate
+
+def test_reference_box_matches_reference_literals(orc32):
+    """The parametric box with the reference preset equals the literal
+    expressions of src/sph.cpp:153-176 bit for bit."""
+    h = F32(0.15)
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, float(h), -9.8, 0.2)
+    box = orc32.box_reference_cpu()
+    dt = F32(0.003)
+    # one particle beyond each wall (isolated: a = g), velocity outward
+    cases = np.array([[0.0, 0.01, 0.0], [-7.99, 1.0, 0.0], [7.99, 1.0, 0.0], [0.0, 1.0, -7.99], [0.0, 1.0, 7.99]],
+                     dtype=F32)
+    vel = np.array([[0, -2, 0], [-3, 0, 0], [3, 0, 0], [0, 0, -3], [0, 0, 3]], dtype=F32)
+    for x0, v0 in zip(cases, vel):
+        p = orc32.particles(x0[None], v0[None])
+        orc32.update(p, s, float(dt), box, nthreads=1)
+        v = v0.copy()
+        v[1] = v[1] + (F32(0) / F32(1) + F32(-9.8)) * dt  # force is 0: a = 0/rho + g
+        x = x0 + v * dt
+        e, off, bw = F32(0.5), F32(0.0001), F32(8)
+        if x[1] < h:
+            x[1] = -x[1] + F32(2) * h + off
+            v[1] = -v[1] * e
+        if x[0] < h - bw:
+            x[0] = -x[0] + F32(2) * (h - bw) + off
+            v[0] = -v[0] * e
+        if x[0] > -h + bw:
+            x[0] = -x[0] + F32(2) * -(h - bw) - off
+            v[0] = -v[0] * e
+        if x[2] < h - bw:
+            x[2] = -x[2] + F32(2) * (h - bw) + off
+            v[2] = -v[2] * e
+        if x[2] > -h + bw:
+            x[2] = -x[2] + F32(2) * -(h - bw) - off
+            v[2] = -v[2] * e
+        assert np.array_equal(p["position"][0], x)
+        assert np.array_equal(p["velocity"][0], v)
+
+
+def test_reflection_is_a_mirror(orc32):
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, 0.1, 0.0, 0.2)
+    box = orc32.box((-1, -1, -1), (1, 1, 1), restitution=0.25, offset=0.0)
+    dt = 0.01
+    vel = np.array([[0, 0, 0], [0, 5.0, 0], [-5.0, 0, 0], [0, 0, 5.0]], dtype=F32)
+    pos = np.array([[0, 0, 0], [0.3, 0.85, 0.3], [-0.85, -0.3, 0.3], [0.3, -0.3, 0.85]], dtype=F32)
+    for x0, v0 in zip(pos[1:], vel[1:]):
+        p = orc32.particles(x0[None], v0[None])
+        orc32.update(p, s, dt, box, nthreads=1)
+        free = x0 + v0 * F32(dt)
+        axis = int(np.argmax(np.abs(v0)))
+        plane = (1 - 0.1) * np.sign(v0[axis])
+        assert p["position"][0, axis] == pytest.approx(2 * plane - free[axis], abs=1e-6)
+        assert p["velocity"][0, axis] == pytest.approx(-0.25 * v0[axis], rel=1e-6)
+
+
+def test_transform_is_translate_times_scale(orc32):
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, 0.15, -9.8, 0.2)
+    box = orc32.box_reference_cpu()
+    pos = np.array([[0.5, 1.0, -0.25], [-3.0, 2.0, 1.5]], dtype=F32)
+    out, xf = _step_by_id(orc32, pos, np.zeros_like(pos), s, 0.003, box)
+    for i in range(2):
+        m = xf[i].reshape(4, 4)  # m[col][row]
+        sc = F32(0.15) / 2
+        np.testing.assert_array_equal(np.diag(m)[:3], [sc, sc, sc])
+        np.testing.assert_array_equal(m[3, :3], out["position"][i])
+        assert m[3, 3] == 1 and np.count_nonzero(m[:3, :3] - np.diag(np.diag(m[:3, :3]))) == 0
+
+
+# ------------------------------------------------------------ sort / table
+
+def test_sort_and_table_invariants(orc32):
+    h = 0.15
+    pos = _axis_cloud(3000, h, 5)
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    p = orc32.particles(pos, ids=np.arange(len(pos)))
+    table = orc32.prepare(p, s, nthreads=3)
+    hashes = p["hash"].astype(np.int64)
+    assert np.all(np.diff(hashes) >= 0)
+    expect_h = brute.h16(brute.ref_cells(pos[p["acceleration"][:, 0].astype(np.int64)], h))
+    assert np.array_equal(hashes, expect_h)
+    t = np.ctypeslib.as_array(table, shape=(262144,))
+    first = {}
+    for i, hv in enumerate(hashes):
+        first.setdefault(int(hv), i)
+    for hv in range(65536):
+        assert t[hv] == first.get(hv, 0xFFFFFFFF)
+    assert np.all(t[65536:] == 0xFFFFFFFF)
+    # stable: ties keep input order
+    ids = p["acceleration"][:, 0]
+    for hv in np.unique(hashes):
+        sel = ids[hashes == hv]
+        assert np.all(np.diff(sel) > 0)
+    orc32.free_table(table)
+
+
+def test_threads_do_not_change_results(orc32):
+    h = 0.15
+    pos = _axis_cloud(2000, h, 9)
+    vel = np.random.default_rng(3).normal(size=pos.shape).astype(F32)
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    box = orc32.box_reference_cpu()
+    a, xa = _step_by_id(orc32, pos, vel, s, 0.003, box, nthreads=1)
+    b, xb = _step_by_id(orc32, pos, vel, s, 0.003, box, nthreads=7)
+    assert a.tobytes() == b.tobytes() and xa.tobytes() == xb.tobytes()
+
+
+def test_fp64_flavour_agrees_with_fp32(orc32, orc64):
+    h = 0.15
+    pos = _axis_cloud(800, h, 13)
+    vel = np.random.default_rng(4).normal(scale=0.1, size=pos.shape).astype(F32)
+    s32 = orc32.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    s64 = orc64.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    a, _ = _step_by_id(orc32, pos, vel, s32, 0.003, orc32.box((-1, -1, -1), (1, 1, 1)))
+    b, _ = _step_by_id(orc64, pos.astype(np.float64), vel.astype(np.float64), s64, 0.003,
+                       orc64.box((-1, -1, -1), (1, 1, 1)))
+    np.testing.assert_allclose(a["density"], b["density"], rtol=1e-5)
+    np.testing.assert_allclose(a["position"], b["position"], rtol=0, atol=1e-5)
