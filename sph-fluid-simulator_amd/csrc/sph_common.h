@@ -18,8 +18,11 @@
 //     double precision the two grid cells are adjacent (clamping to the grid
 //     keeps adjacency), so every reference neighbour is a candidate.
 //   * Multiplicity: when the 27 buckets around C_i are pairwise distinct and
-//     x_i is not within 2^-9 cell widths of a reference cell boundary, every
-//     neighbour j has C_j adjacent to C_i and m_ij == 1 ("fast" particle).
+//     q_i = fl(x_i / h) is at least eta(q_i) = 2^-21 + 2^-23 (|q_i| + 1) from
+//     every reference cell boundary (nonzero integers), every neighbour j
+//     has C_j adjacent to C_i and m_ij == 1 ("fast" particle).  Proof:
+//     |x_j - x_i| < h (1 + 2^-22) and each quotient carries a relative error
+//     below 2^-24, so |q_j - q_i| < 1 + 2^-22 + 2^-24 (2 |q_i| + 2) <= 1 + eta.
 //     Otherwise the particle is flagged "slow" and m_ij is counted from the
 //     27 buckets for every neighbour.  The bucket test is exact: two offsets
 //     collide iff the six 16-bit vectors
@@ -47,7 +50,6 @@ constexpr int kGridMaxY = 1024;
 constexpr int kGridMaxZ = 1024;
 constexpr uint32_t kSlowBit = 0x80000000u;
 constexpr double kCellMargin = 1.0 / 1024.0;  // H = h (1 + 2^-10)
-constexpr float kNearEta = 1.0f / 512.0f;     // boundary band, reference cells
 constexpr float kQMax = 8192.0f;              // |x/h| beyond this: always slow
 
 // Per-scene constants used by every pass.  Derived on the host from the
@@ -84,15 +86,17 @@ SPH_HD inline int trunc_to_int(float q)
     return (int)0x80000000u;
 }
 
-// True when q = x / h lies within kNearEta of a reference cell boundary
-// (a nonzero integer), or |q| >= kQMax, or q is NaN.
+// True when q = x / h lies within eta(q) = 2^-21 + 2^-23 (|q| + 1) of a
+// reference cell boundary (a nonzero integer), or |q| >= kQMax, or q is NaN.
+// Both distances below are exact in float (Sterbenz) for |q| >= 1.
 SPH_HD inline bool near_cell_boundary(float q)
 {
     float a = q < 0.0f ? -q : q;
     if (!(a < kQMax))
         return true;
     float k = (float)(int)a;  // exact for a < 2^13
-    return (k >= 1.0f && a - k < kNearEta) || (k + 1.0f - a < kNearEta);
+    float eta = 4.76837158203125e-07f + 1.1920928955078125e-07f * (a + 1.0f);  // 2^-21, 2^-23
+    return (k >= 1.0f && a - k < eta) || (k + 1.0f - a < eta);
 }
 
 // True when two of the 27 buckets around reference cell c coincide.

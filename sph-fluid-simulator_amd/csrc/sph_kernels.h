@@ -22,6 +22,11 @@
 namespace sph {
 
 constexpr int kBlock = 256;           // threads per block, particle kernels
+constexpr int kStencilBlock = 128;    // threads per block, density / force
+#ifndef SPH_STAGE_CAP
+#define SPH_STAGE_CAP 0
+#endif
+constexpr uint32_t kStageCap = SPH_STAGE_CAP;  // positions staged in LDS per block (0: off)
 constexpr int kScanItems = 16;        // cells per thread in the scan
 constexpr uint32_t kScanChunk = kBlock * kScanItems;  // cells per scan block
 constexpr uint32_t kStableCellMax = 512;  // larger cells keep arrival order

@@ -328,13 +328,69 @@ void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c
 }
 
 // ------------------------------------------------------------ stencil walk
+// The stencil kernels run blocks of kStencilBlock consecutive particles of
+// the cell-ordered arrays.  Their cells lie in one linear range [L0, L1], so
+// for each of the 9 (dy, dz) rows the candidates of every particle of the
+// block lie in one contiguous "band" of the cell-ordered arrays: cells
+// [L0 - 1 + off, L1 + 1 + off], off = nx (dy + ny dz).  The block copies the
+// 9 bands of positions into LDS once (coalesced) and every lane then walks
+// its own three-cell x-run of each row out of LDS.  A block whose bands
+// exceed kStageCap walks global memory instead (same order, same results).
+struct StageInfo {
+    uint32_t src[9];   // first cell-ordered index of band r
+    uint32_t dst[10];  // LDS offset of band r; dst[9] = total staged
+};
+
+// Fills `info` and, when the bands fit, `sh`.  Every thread of the block
+// must call it (it synchronises); returns whether the bands are staged.
+__device__ inline bool stage_bands(const float4 *__restrict__ pos, const uint32_t *__restrict__ cellc,
+                                   const uint32_t *__restrict__ cstart, uint32_t first, uint32_t last,
+                                   const StepParams &P, StageInfo &info, float4 *__restrict__ sh)
+{
+    if (threadIdx.x < 9) {
+        const int r = threadIdx.x;
+        const int dz = r / 3 - 1, dy = r % 3 - 1;
+        const int64_t L0 = linear_cell(P, cellc[first]);
+        const int64_t L1 = linear_cell(P, cellc[last]);
+        const int64_t off = (int64_t)P.nx * ((int64_t)dy + (int64_t)P.ny * dz);
+        const int64_t lo = P.cell_base;
+        const int64_t hi = (int64_t)P.cell_base + (int64_t)P.nx * P.ny * P.nz;
+        int64_t a = L0 - 1 + off, b = L1 + 2 + off;
+        a = a < lo ? lo : (a > hi ? hi : a);
+        b = b < lo ? lo : (b > hi ? hi : b);
+        if (b < a)
+            b = a;
+        const uint32_t ca = cstart[a];
+        info.src[r] = ca;
+        info.dst[r + 1] = cstart[b] - ca;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        info.dst[0] = 0;
+        for (int r = 0; r < 9; ++r)
+            info.dst[r + 1] += info.dst[r];
+    }
+    __syncthreads();
+    const bool staged = info.dst[9] <= kStageCap;
+    if (staged) {
+        for (int r = 0; r < 9; ++r) {
+            const uint32_t src = info.src[r], d0 = info.dst[r], len = info.dst[r + 1] - d0;
+            for (uint32_t k = threadIdx.x; k < len; k += blockDim.x)
+                sh[d0 + k] = pos[src + k];
+        }
+    }
+    __syncthreads();
+    return staged;
+}
+
 // Calls body(j, q) for every candidate j != s in the 3x3x3 grid-cell block
-// around the particle; the x-run of three cells in each of the 9 rows is one
-// contiguous range of the cell-ordered arrays.
-template <class Body>
-__device__ inline void for_each_candidate(const float4 *__restrict__ pos,
-                                          const uint32_t *__restrict__ cstart, uint32_t s,
-                                          uint32_t cc, const StepParams &P, Body body)
+// around the particle (q = its position record), reading positions from LDS
+// when STAGED.  The x-run of three cells in each of the 9 rows is one
+// contiguous range [b, e) of the cell-ordered arrays, inside band r.
+template <bool STAGED, class Body>
+__device__ inline void walk_candidates(const float4 *__restrict__ pos, const float4 *__restrict__ sh,
+                                       const StageInfo &info, const uint32_t *__restrict__ cstart,
+                                       uint32_t s, uint32_t cc, const StepParams &P, Body body)
 {
     const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc);
     const int x0 = gx > 0 ? gx - 1 : 0;
@@ -350,11 +406,19 @@ __device__ inline void for_each_candidate(const float4 *__restrict__ pos,
             const uint32_t row = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
             const uint32_t b = cstart[row + (uint32_t)x0];
             const uint32_t e = cstart[row + (uint32_t)x1 + 1u];
-            for (uint32_t j = b; j < e; ++j) {
-                if (j == s)
-                    continue;
-                body(j, pos[j]);
+            const int r = (dz + 1) * 3 + (dy + 1);
+            const uint32_t shift = STAGED ? info.dst[r] - info.src[r] : 0u;
+            uint32_t j = b;
+            for (; j + 1u < e; j += 2u) {  // two candidates in flight
+                const float4 q0 = STAGED ? sh[j + shift] : pos[j];
+                const float4 q1 = STAGED ? sh[j + 1u + shift] : pos[j + 1u];
+                if (j != s)
+                    body(j, q0);
+                if (j + 1u != s)
+                    body(j + 1u, q1);
             }
+            if (j < e && j != s)
+                body(j, STAGED ? sh[j + shift] : pos[j]);
         }
     }
 }
@@ -366,6 +430,23 @@ __device__ inline float dist2_ref(float4 pi, float4 q, float &dx, float &dy, flo
     dy = q.y - pi.y;
     dz = q.z - pi.z;
     return dx * dx + dy * dy + dz * dz;
+}
+
+// Correctly rounded sqrt for d2 >= 2^-96 (any d2 a neighbour pair can have
+// short of coincident points; d2 = 0 also comes out exact): v_sqrt_f32 is
+// within 1 ulp, and the neighbouring candidate whose residual
+// d2 - s * s' changes sign is the correctly rounded root (the tiny-input
+// rescaling of the library sequence is not needed in this range).
+__device__ inline float sqrt_rn(float x)
+{
+    float s = __builtin_amdgcn_sqrtf(x);
+    float down = __uint_as_float(__float_as_uint(s) - 1u);
+    float up = __uint_as_float(__float_as_uint(s) + 1u);
+    float r_down = fmaf(-down, s, x);
+    float r_up = fmaf(-up, s, x);
+    s = (r_down <= 0.0f) ? down : s;
+    s = (r_up > 0.0f) ? up : s;
+    return s;
 }
 
 struct RefCell {
@@ -381,13 +462,22 @@ __device__ inline RefCell ref_cell(float4 p, const StepParams &P)
 // MODE 0: density + pressure.  MODE 1: also the neighbour count per
 // particle.  MODE 2: also the neighbour ids (multiplicity-repeated).
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_density(
+__global__ __launch_bounds__(kStencilBlock) void k_density(
     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
     const uint32_t *__restrict__ cellc, const uint32_t *__restrict__ cstart,
     float *__restrict__ rho, float2 *__restrict__ pinv, uint32_t n, StepParams P,
     NeighborDump dump)
 {
-    uint32_t s = particle_index();
+    __shared__ float4 sh[kStageCap > 0 ? kStageCap : 1];
+    __shared__ StageInfo info;
+    const uint32_t first = xcd_block() * kStencilBlock;
+    if (first >= n)
+        return;  // whole block
+    const uint32_t last = min(first + (uint32_t)kStencilBlock, n) - 1u;
+    bool staged = false;
+    if constexpr (kStageCap > 0)
+        staged = stage_bands(pos, cellc, cstart, first, last, P, info, sh);
+    const uint32_t s = first + threadIdx.x;
     if (s >= n)
         return;
     const float4 pi = pos[s];
@@ -397,7 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_density(
     float acc = 0.0f;
     uint32_t cnt = 0;
     uint64_t w = (MODE == 2) ? dump.off[s] : 0;
-    for_each_candidate(pos, cstart, s, cc, P, [&](uint32_t j, float4 q) {
+    auto body = [&](uint32_t j, float4 q) {
         float dx, dy, dz;
         float d2 = dist2_ref(pi, q, dx, dy, dz);
         if (d2 < P.h2) {
@@ -413,7 +503,11 @@ __global__ __launch_bounds__(kBlock) void k_density(
                     dump.out[w++] = id;
             }
         }
-    });
+    };
+    if (staged)
+        walk_candidates<true>(pos, sh, info, cstart, s, cc, P, body);
+    else
+        walk_candidates<false>(pos, sh, info, cstart, s, cc, P, body);
     const float r = acc + P.self_dens;
     rho[s] = r;
     pinv[s] = make_float2(P.gas * (r - P.rho0), 1.0f / r);
@@ -425,7 +519,7 @@ void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c
                     const StepParams &P, const NeighborDump *dump)
 {
     NeighborDump d = dump ? *dump : NeighborDump{nullptr, nullptr, nullptr};
-    dim3 g(blocks_for(n)), b(kBlock);
+    dim3 g(blocks_for(n, kStencilBlock)), b(kStencilBlock);
     if (!dump)
         hipLaunchKernelGGL(k_density<0>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
                            a.pinvB, n, P, d);
@@ -486,14 +580,23 @@ __device__ inline void integrate_and_walls(float4 &x, float4 &v, float fx, float
 }
 
 template <bool WRITE_FORCE, bool NEXT_KEYS>
-__global__ __launch_bounds__(kBlock) void k_force_integrate(
+__global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
     const float4 *__restrict__ posB, const float4 *__restrict__ velB,
     const uint32_t *__restrict__ cellcB, const uint32_t *__restrict__ cstart,
     const float *__restrict__ rhoB, const float2 *__restrict__ pinvB, float4 *__restrict__ posA,
     float4 *__restrict__ velA, uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA,
     uint32_t *__restrict__ counts, float4 *__restrict__ force_out, uint32_t n, StepParams P)
 {
-    uint32_t s = particle_index();
+    __shared__ float4 sh[kStageCap > 0 ? kStageCap : 1];
+    __shared__ StageInfo info;
+    const uint32_t first = xcd_block() * kStencilBlock;
+    if (first >= n)
+        return;  // whole block
+    const uint32_t last = min(first + (uint32_t)kStencilBlock, n) - 1u;
+    bool staged = false;
+    if constexpr (kStageCap > 0)
+        staged = stage_bands(posB, cellcB, cstart, first, last, P, info, sh);
+    const uint32_t s = first + threadIdx.x;
     if (s >= n)
         return;
     float4 pi = posB[s];
@@ -504,15 +607,18 @@ __global__ __launch_bounds__(kBlock) void k_force_integrate(
     const float rho_i = rhoB[s];
     const float p_i = pinvB[s].x;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for_each_candidate(posB, cstart, s, cc, P, [&](uint32_t j, float4 q) {
+    auto body = [&](uint32_t j, float4 q) {
         float dx, dy, dz;
         float d2 = dist2_ref(pi, q, dx, dy, dz);
         if (d2 < P.h2) {
             int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
             const float2 pj = pinvB[j];
             const float4 vj = velB[j];
-            const float rinv = __builtin_amdgcn_rsqf(d2);
-            const float r = d2 * rinv;
+            // r = sqrt(d2) correctly rounded and hr = h - r exactly as the
+            // reference forms them: for neighbours near the support edge
+            // h - r cancels, and any error in r would be amplified.
+            const float r = sqrt_rn(d2);
+            const float rinv = __builtin_amdgcn_rcpf(r);
             const float hr = P.h - r;
             float cp = P.neg_m_spg * (p_i + pj.x) * (0.5f * pj.y) * (hr * hr) * rinv;
             float cv = P.mu_m_lap * hr * pj.y;
@@ -525,7 +631,11 @@ __global__ __launch_bounds__(kBlock) void k_force_integrate(
             fy = fmaf(cv, vj.y - vi.y, fy);
             fz = fmaf(cv, vj.z - vi.z, fz);
         }
-    });
+    };
+    if (staged)
+        walk_candidates<true>(posB, sh, info, cstart, s, cc, P, body);
+    else
+        walk_candidates<false>(posB, sh, info, cstart, s, cc, P, body);
     if (WRITE_FORCE)
         force_out[s] = make_float4(fx, fy, fz, 0.0f);
     integrate_and_walls(pi, vi, fx, fy, fz, rho_i, P);
@@ -544,7 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_force_integrate(
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
                             uint32_t n, const StepParams &P, float4 *force_out, bool next_keys)
 {
-    dim3 g(blocks_for(n)), b(kBlock);
+    dim3 g(blocks_for(n, kStencilBlock)), b(kStencilBlock);
 #define SPH_LAUNCH_FI(WF, NK)                                                                   \
     hipLaunchKernelGGL((k_force_integrate<WF, NK>), g, b, 0, st, a.posB, a.velB, a.cellcB,      \
                        c.cstart, a.rhoB, a.pinvB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,  \

@@ -96,15 +96,30 @@ def test_slow_flag_covers_every_bucket_collision(sph):
     assert hits > 50
 
 
+def _eta(q):
+    return 2.0 ** -21 + 2.0 ** -23 * (abs(q) + 1.0)
+
+
 def test_near_boundary_particles_are_slow(sph):
+    """Positions whose quotient x/h lies within eta(q) = 2^-21 + 2^-23 (|q|+1)
+    of a nonzero integer are on the exact-multiplicity path; positions well
+    outside that band (same row, same bucket structure) are not."""
     h = F32(0.05)
-    for k in (1, 2, 7, -1, -3, 100):
-        b = F32(k) * h
-        for eps in (0.0, 1e-7, 1e-5, 5e-5):  # all within 2^-9 h of the boundary
+    y, z = F32(0.61) * h + F32(3.3), F32(5.43)
+    for k in (1, 2, 7, -1, -3, 100, 1000):
+        for frac in (0.0, 0.25, 0.9):
             for s in (-1, 1):
-                x = F32(b + F32(s * eps))
-                _, slow, _ = sph.classify_host((x, F32(0.61) * h + F32(3.3), F32(5.43)), float(h))
-                assert slow, (k, eps, s)
+                q = F32(k + s * frac * _eta(k))
+                x = F32(q * h)
+                qx = float(x / h)
+                if abs(qx - k) >= _eta(k):
+                    continue  # rounding moved it out of the band
+                _, slow, _ = sph.classify_host((x, y, z), float(h))
+                assert slow, (k, frac, s)
+        # far from the boundary: only a bucket collision could make it slow
+        x = F32((k + 0.5) * h)
+        _, slow, cell = sph.classify_host((x, y, z), float(h))
+        assert slow == brute.buckets_collide_brute(cell), (k, cell)
 
 
 def _fast_path_holds(sph, pos, h):
@@ -143,7 +158,7 @@ def test_fast_path_theorem_boundary_adversarial(sph):
         b = F32(k) * h
         y, z = rng.uniform(-1, 1, size=2).astype(F32)
         dx = F32(rng.uniform(0.99, 1.0)) * h
-        x0 = F32(b - F32(rng.uniform(0, 2e-6)))
+        x0 = F32(b - F32(rng.uniform(0, 2e-6)) * rng.choice([1e-3, 1.0, 30.0]))
         pts.append((x0, y, z))
         pts.append((F32(x0 + dx), y, z))
     pos = np.array(pts, dtype=F32)
