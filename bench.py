@@ -1,17 +1,21 @@
 """Benchmark of the SPH step on one MI355X.
 
-Workload (the metric's configuration): 1,000,000-particle dam break
+Workload (the metric's configuration): the 1,000,000-particle dam break
 (100^3 jittered lattice, spacing 0.5 h, h = 0.03, dt = 0.001, mass 1.6e-4,
-box [-2,2] x [-1,1]^2), state resident in HBM.  A step is one pass of the
-SPH operator over all particles (scan, scatter, reorder, density,
-force + integrate).  `value` is particle-updates/s = N * steps / wall time of
-the timed steps, bracketed by device synchronisation.
+box [-2,2] x [-1,1]^2), state resident in HBM.  A step is one pass of the SPH
+operator over all particles (scan, scatter, reorder, density, force +
+integrate).  The timed job is the workload's first K steps (default its full
+1000) from the initial state, replayed as HIP graphs after W untimed warm-up
+steps on the same context (re-uploaded before the timed run); `value` is
+particle-updates/s = N * K / wall time, bracketed by device synchronisation.
 
 The JSON line also carries
   roofline      the dominant kernel's algorithmic bytes per launch (the
                 per-particle figure of its passes times N) over its mean
-                duration, timed with HIP events on the kernel's stream,
-                against the 8 TB/s HBM peak
+                duration, timed with HIP events on the kernel's stream over
+                the same K-step job, against the 8 TB/s HBM peak; `traffic`
+                is the measured HBM bytes per launch when a counter summary
+                for the kernel is present under profiles/
   cpu_baseline  the oracle (CPU restatement of the reference's
                 multithreaded update()) timed on this host over a few steps
                 of the same workload
@@ -52,7 +56,7 @@ def cpu_threads():
     return min(os.cpu_count() or 1, 16)
 
 
-def cpu_baseline(sc, budget_s=12.0, max_steps=20):
+def cpu_baseline(sc, budget_s=15.0, max_steps=20):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from oracle_binding import Oracle
 
@@ -78,14 +82,26 @@ def cpu_baseline(sc, budget_s=12.0, max_steps=20):
     }
 
 
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from a committed counter summary
+    (profiles/traffic_<kernel>_workload2.json), or None."""
+    path = os.path.join(REPO, "profiles", f"traffic_{kernel}_workload2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", type=int, default=2)
-    ap.add_argument("--profile-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event pass")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run the copy kernel over this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -98,8 +114,9 @@ def main():
     b = sph_amd.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset)
     ctx = sph_amd.Context(sc.n, s, b, sc.dt)
     ctx.upload(sc.pos, sc.vel)
-
     ctx.step(args.warmup)
+    ctx.sync()
+    ctx.upload(sc.pos, sc.vel)  # the timed job starts from the initial state
     ctx.sync()
     t0 = time.perf_counter()
     ctx.step(args.steps)
@@ -109,25 +126,32 @@ def main():
     value = sc.n * args.steps / elapsed
     log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value / 1e9:.3f} G particle-updates/s")
 
-    prof = ctx.profile(args.profile_steps)
-    log("per-kernel ms (HIP events): " + ", ".join(f"{k}={v:.4f}" for k, v in prof.items()))
-    dom = max(prof, key=prof.get)
-    ach = ALG_BYTES[dom] * sc.n / (prof[dom] * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(ach, 2),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(ach / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "kernel": dom,
-        "kernel_ms": round(prof[dom], 5),
-        "alg_bytes_per_particle": ALG_BYTES[dom],
-        "step_alg_GBs": round(ALG_BYTES_STEP * value / 1e9, 2),
-        "step_frac": round(ALG_BYTES_STEP * value / 1e9 / HBM_PEAK_GBS, 4),
-        "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()},
-    }
+    roofline = None
+    if not args.no_profile:
+        ctx.upload(sc.pos, sc.vel)  # same job again, with events around every kernel
+        prof = ctx.profile(args.steps)
+        log("per-kernel mean ms (HIP events): " + ", ".join(f"{k}={v:.4f}" for k, v in prof.items()))
+        dom = max(prof, key=prof.get)
+        ach = ALG_BYTES[dom] * sc.n / (prof[dom] * 1e-3) / 1e9
+        roofline = {
+            "bound": "hbm",
+            "achieved": round(ach, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(dom),
+            "kernel": dom,
+            "kernel_ms": round(prof[dom], 5),
+            "alg_bytes_per_particle": ALG_BYTES[dom],
+            "step_alg_GBs": round(ALG_BYTES_STEP * value / 1e9, 2),
+            "step_frac": round(ALG_BYTES_STEP * value / 1e9 / HBM_PEAK_GBS, 4),
+            "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()},
+        }
     ctx.close()
+
+    if args.copy_bytes > 0:
+        ms = sph_amd.copy_bench(args.copy_bytes, 10)
+        log(f"copy kernel: {args.copy_bytes} B in {ms:.4f} ms = {2 * args.copy_bytes / ms / 1e6:.1f} GB/s (read+write)")
 
     cpu = None
     if not args.no_cpu:
@@ -150,7 +174,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{sc.name}: {sc.n} particles, 100^3 jittered lattice dam break, h={sc.h}, "
-                        f"dt={sc.dt}, box [-2,2]x[-1,1]^2, resident in HBM",
+                        f"dt={sc.dt}, box [-2,2]x[-1,1]^2, steps 1..{args.steps} of the job, resident in HBM",
             "particles": sc.n,
             "parallelism": "single GPU",
         },

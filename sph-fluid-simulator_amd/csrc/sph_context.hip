@@ -185,7 +185,8 @@ int sph_hip_upload(sph_ctx *c, const float *pos, const float *vel)
         return hip_fail(e, "upload");
     launch_fill_u32(c->stream, cells.counts, 0u, cells.ncells_padded);
     launch_fill_u32(c->stream, (uint32_t *)a.rhoB, 0u, n);
-    launch_fill_u32(c->stream, (uint32_t *)a.pinvB, 0u, 2ull * n);
+    launch_fill_u32(c->stream, (uint32_t *)a.posR, 0u, 4ull * n);
+    launch_fill_u32(c->stream, (uint32_t *)a.velR, 0u, 4ull * n);
     launch_fill_u32(c->stream, (uint32_t *)a.posB, 0u, 4ull * n);
     if (c->keep_force)
         launch_fill_u32(c->stream, (uint32_t *)c->w.force.ptr, 0u, 4ull * n);
@@ -245,7 +246,7 @@ int sph_hip_download(sph_ctx *c, float *pos, float *vel, float *rho, float *pres
     ParticleArrays a = c->w.particles();
     std::vector<float4> p4, v4;
     std::vector<float> r;
-    std::vector<float2> pi;
+    std::vector<float4> pi;
     hipError_t e = hipSuccess;
     if (pos) {
         p4.resize(n);
@@ -261,7 +262,7 @@ int sph_hip_download(sph_ctx *c, float *pos, float *vel, float *rho, float *pres
     }
     if (e == hipSuccess && pressure) {
         pi.resize(n);
-        e = hipMemcpyAsync(pi.data(), a.pinvB, 8ull * n, hipMemcpyDeviceToHost, c->stream);
+        e = hipMemcpyAsync(pi.data(), a.velR, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, c->stream);
     }
     if (e == hipSuccess)
         e = hipStreamSynchronize(c->stream);
@@ -283,7 +284,7 @@ int sph_hip_download(sph_ctx *c, float *pos, float *vel, float *rho, float *pres
         if (rho)
             rho[i] = r[i];
         if (pressure)
-            pressure[i] = pi[i].x;
+            pressure[i] = pi[i].w;
     }
     return SPH_OK;
 }

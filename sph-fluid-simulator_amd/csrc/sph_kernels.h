@@ -9,8 +9,11 @@
 //     rankA  u32     arrival rank of the particle inside its grid cell
 //   "B" arrays: the same particles in grid-cell order (stable by A index)
 //     posB, velB, cellcB  as above
-//     rhoB   f32     density            (written by the density pass)
-//     pinvB  float2  pressure, 1 / density
+//     rhoB   f32     density                          (density pass)
+//     posR   float4  x, y, z, 0.5 / density           (density pass)
+//     velR   float4  vx, vy, vz, pressure             (density pass)
+//   posR / velR are what the force pass gathers: one record for the
+//   candidate filter and one more per neighbour.
 //   cells: counts[ncells] (zero between steps), cstart[ncells + 1]
 #pragma once
 
@@ -27,6 +30,7 @@ constexpr int kStencilBlock = 128;    // threads per block, density / force
 #define SPH_STAGE_CAP 0
 #endif
 constexpr uint32_t kStageCap = SPH_STAGE_CAP;  // positions staged in LDS per block (0: off)
+constexpr int kForceChunk = 16;       // candidates filtered per bitmask in the force pass
 constexpr int kScanItems = 16;        // cells per thread in the scan
 constexpr uint32_t kScanChunk = kBlock * kScanItems;  // cells per scan block
 constexpr uint32_t kStableCellMax = 512;  // larger cells keep arrival order
@@ -35,7 +39,7 @@ struct ParticleArrays {
     float4 *posA, *velA, *posB, *velB;
     uint32_t *cellcA, *rankA, *cellcB, *perm;
     float *rhoB;
-    float2 *pinvB;
+    float4 *posR, *velR;
 };
 
 struct CellArrays {

@@ -465,8 +465,8 @@ template <int MODE>
 __global__ __launch_bounds__(kStencilBlock) void k_density(
     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
     const uint32_t *__restrict__ cellc, const uint32_t *__restrict__ cstart,
-    float *__restrict__ rho, float2 *__restrict__ pinv, uint32_t n, StepParams P,
-    NeighborDump dump)
+    float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, uint32_t n,
+    StepParams P, NeighborDump dump)
 {
     __shared__ float4 sh[kStageCap > 0 ? kStageCap : 1];
     __shared__ StageInfo info;
@@ -509,8 +509,11 @@ __global__ __launch_bounds__(kStencilBlock) void k_density(
     else
         walk_candidates<false>(pos, sh, info, cstart, s, cc, P, body);
     const float r = acc + P.self_dens;
+    const float p = P.gas * (r - P.rho0);
     rho[s] = r;
-    pinv[s] = make_float2(P.gas * (r - P.rho0), 1.0f / r);
+    posR[s] = make_float4(pi.x, pi.y, pi.z, 0.5f / r);
+    const float4 vs = vel[s];
+    velR[s] = make_float4(vs.x, vs.y, vs.z, p);
     if (MODE >= 1)
         dump.count[s] = cnt;
 }
@@ -522,13 +525,13 @@ void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c
     dim3 g(blocks_for(n, kStencilBlock)), b(kStencilBlock);
     if (!dump)
         hipLaunchKernelGGL(k_density<0>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.pinvB, n, P, d);
+                           a.posR, a.velR, n, P, d);
     else if (!dump->out)
         hipLaunchKernelGGL(k_density<1>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.pinvB, n, P, d);
+                           a.posR, a.velR, n, P, d);
     else
         hipLaunchKernelGGL(k_density<2>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.pinvB, n, P, d);
+                           a.posR, a.velR, n, P, d);
 }
 
 // ----------------------------------------------------- force + integrate
@@ -579,63 +582,92 @@ __device__ inline void integrate_and_walls(float4 &x, float4 &v, float fx, float
     }
 }
 
+// Candidates are filtered kForceChunk at a time into a per-lane bitmask;
+// the force terms then run only over the set bits, so a wave pays for the
+// most neighbours any of its lanes has in the chunk, not for every
+// candidate some lane accepts.
 template <bool WRITE_FORCE, bool NEXT_KEYS>
 __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
     const float4 *__restrict__ posB, const float4 *__restrict__ velB,
+    const float4 *__restrict__ posR, const float4 *__restrict__ velR,
     const uint32_t *__restrict__ cellcB, const uint32_t *__restrict__ cstart,
-    const float *__restrict__ rhoB, const float2 *__restrict__ pinvB, float4 *__restrict__ posA,
-    float4 *__restrict__ velA, uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA,
-    uint32_t *__restrict__ counts, float4 *__restrict__ force_out, uint32_t n, StepParams P)
+    const float *__restrict__ rhoB, float4 *__restrict__ posA, float4 *__restrict__ velA,
+    uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA, uint32_t *__restrict__ counts,
+    float4 *__restrict__ force_out, uint32_t n, StepParams P)
 {
-    __shared__ float4 sh[kStageCap > 0 ? kStageCap : 1];
-    __shared__ StageInfo info;
     const uint32_t first = xcd_block() * kStencilBlock;
     if (first >= n)
         return;  // whole block
-    const uint32_t last = min(first + (uint32_t)kStencilBlock, n) - 1u;
-    bool staged = false;
-    if constexpr (kStageCap > 0)
-        staged = stage_bands(posB, cellcB, cstart, first, last, P, info, sh);
     const uint32_t s = first + threadIdx.x;
     if (s >= n)
         return;
-    float4 pi = posB[s];
-    float4 vi = velB[s];
+    float4 pi = posR[s];  // w = 0.5 / rho_i
+    const float4 vr = velR[s];  // w = p_i
+    float4 vi = make_float4(vr.x, vr.y, vr.z, velB[s].w);  // w = particle id
     const uint32_t cc = cellcB[s];
     const bool slow = (cc & kSlowBit) != 0u;
     const RefCell ci = ref_cell(pi, P);
     const float rho_i = rhoB[s];
-    const float p_i = pinvB[s].x;
+    const float p_i = vr.w;
+    const float mu2_m_lap = 2.0f * P.mu_m_lap;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    auto body = [&](uint32_t j, float4 q) {
-        float dx, dy, dz;
-        float d2 = dist2_ref(pi, q, dx, dy, dz);
-        if (d2 < P.h2) {
-            int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
-            const float2 pj = pinvB[j];
-            const float4 vj = velB[j];
-            // r = sqrt(d2) correctly rounded and hr = h - r exactly as the
-            // reference forms them: for neighbours near the support edge
-            // h - r cancels, and any error in r would be amplified.
-            const float r = sqrt_rn(d2);
-            const float rinv = __builtin_amdgcn_rcpf(r);
-            const float hr = P.h - r;
-            float cp = P.neg_m_spg * (p_i + pj.x) * (0.5f * pj.y) * (hr * hr) * rinv;
-            float cv = P.mu_m_lap * hr * pj.y;
-            cp *= (float)m;
-            cv *= (float)m;
-            fx = fmaf(cp, dx, fx);
-            fy = fmaf(cp, dy, fy);
-            fz = fmaf(cp, dz, fz);
-            fx = fmaf(cv, vj.x - vi.x, fx);
-            fy = fmaf(cv, vj.y - vi.y, fy);
-            fz = fmaf(cv, vj.z - vi.z, fz);
+
+    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc);
+    const int x0 = gx > 0 ? gx - 1 : 0;
+    const int x1 = gx + 1 < P.nx ? gx + 1 : P.nx - 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = gz + dz;
+        if (z < 0 || z >= P.nz)
+            continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = gy + dy;
+            if (y < 0 || y >= P.ny)
+                continue;
+            const uint32_t row = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
+            const uint32_t b = cstart[row + (uint32_t)x0];
+            const uint32_t e = cstart[row + (uint32_t)x1 + 1u];
+            for (uint32_t jj = b; jj < e; jj += kForceChunk) {
+                uint32_t mask = 0u;
+#pragma unroll
+                for (int k = 0; k < kForceChunk; ++k) {
+                    const uint32_t j = jj + (uint32_t)k;
+                    const float4 q = posR[j < e ? j : e - 1u];
+                    float dx, dy_, dz_;
+                    const float d2 = dist2_ref(pi, q, dx, dy_, dz_);
+                    const bool pass = (j < e) && (j != s) && (d2 < P.h2);
+                    mask |= pass ? (1u << k) : 0u;
+                }
+                while (mask != 0u) {
+                    const uint32_t k = (uint32_t)__builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const uint32_t j = jj + k;
+                    const float4 q = posR[j];  // w = 0.5 / rho_j
+                    const float4 w = velR[j];  // w = p_j
+                    float dx, dy_, dz_;
+                    const float d2 = dist2_ref(pi, q, dx, dy_, dz_);
+                    const int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z,
+                                                             __float_as_uint(posB[j].w) & 0xFFFFu)
+                                       : 1;
+                    // r = sqrt(d2) correctly rounded and hr = h - r exactly as
+                    // the reference forms them: near the support edge h - r
+                    // cancels and any error in r would be amplified.
+                    const float r = sqrt_rn(d2);
+                    const float rinv = __builtin_amdgcn_rcpf(r);
+                    const float hr = P.h - r;
+                    float cp = P.neg_m_spg * (p_i + w.w) * q.w * (hr * hr) * rinv;
+                    float cv = mu2_m_lap * hr * q.w;
+                    cp *= (float)m;
+                    cv *= (float)m;
+                    fx = fmaf(cp, dx, fx);
+                    fy = fmaf(cp, dy_, fy);
+                    fz = fmaf(cp, dz_, fz);
+                    fx = fmaf(cv, w.x - vi.x, fx);
+                    fy = fmaf(cv, w.y - vi.y, fy);
+                    fz = fmaf(cv, w.z - vi.z, fz);
+                }
+            }
         }
-    };
-    if (staged)
-        walk_candidates<true>(posB, sh, info, cstart, s, cc, P, body);
-    else
-        walk_candidates<false>(posB, sh, info, cstart, s, cc, P, body);
+    }
     if (WRITE_FORCE)
         force_out[s] = make_float4(fx, fy, fz, 0.0f);
     integrate_and_walls(pi, vi, fx, fy, fz, rho_i, P);
@@ -656,8 +688,8 @@ void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellA
 {
     dim3 g(blocks_for(n, kStencilBlock)), b(kStencilBlock);
 #define SPH_LAUNCH_FI(WF, NK)                                                                   \
-    hipLaunchKernelGGL((k_force_integrate<WF, NK>), g, b, 0, st, a.posB, a.velB, a.cellcB,      \
-                       c.cstart, a.rhoB, a.pinvB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,  \
+    hipLaunchKernelGGL((k_force_integrate<WF, NK>), g, b, 0, st, a.posB, a.velB, a.posR, a.velR, \
+                       a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,  \
                        force_out, n, P)
     if (force_out && next_keys)
         SPH_LAUNCH_FI(true, true);
@@ -740,7 +772,7 @@ struct AosOut {
 // input order).
 __global__ void k_pack_out(const uint32_t *__restrict__ perm16, const uint32_t *__restrict__ cstart16,
                            const float4 *__restrict__ posB, const float4 *__restrict__ velB,
-                           const float *__restrict__ rhoB, const float2 *__restrict__ pinvB,
+                           const float *__restrict__ rhoB, const float4 *__restrict__ velR,
                            const float4 *__restrict__ force, const float4 *__restrict__ posA,
                            const float4 *__restrict__ velA, const float *__restrict__ aos_in,
                            float *__restrict__ aos_out, float *__restrict__ xf_out, uint32_t n,
@@ -778,7 +810,7 @@ __global__ void k_pack_out(const uint32_t *__restrict__ perm16, const uint32_t *
     dst[10] = f.y;
     dst[11] = f.z;
     dst[12] = rhoB[s];
-    dst[13] = pinvB[s].x;
+    dst[13] = velR[s].w;
     dst[14] = __uint_as_float(h);  // uint16 hash + zero padding (little endian)
     if (xf_out)
         write_transform(xf_out + (size_t)d * 16, x.x, x.y, x.z, sc);
@@ -789,7 +821,7 @@ void launch_pack_out(hipStream_t st, const ParticleArrays &a, const CellArrays &
                      const StepParams &P, void *aos_out, float *xf_out)
 {
     hipLaunchKernelGGL(k_pack_out, dim3(blocks_for(n)), dim3(kBlock), 0, st, perm16, c16.cstart,
-                       a.posB, a.velB, a.rhoB, a.pinvB, force, a.posA, a.velA,
+                       a.posB, a.velB, a.rhoB, a.velR, force, a.posA, a.velA,
                        (const float *)aos_in, (float *)aos_out, xf_out, n, P.xf_scale);
 }
 
@@ -797,7 +829,7 @@ void launch_pack_out(hipStream_t st, const ParticleArrays &a, const CellArrays &
 // and pressure belong to the last step (B slot s is the same particle).
 __global__ void k_pack_state(const float4 *__restrict__ posA, const float4 *__restrict__ velA,
                              const float4 *__restrict__ posB, const float *__restrict__ rhoB,
-                             const float2 *__restrict__ pinvB, const float4 *__restrict__ force,
+                             const float4 *__restrict__ velR, const float4 *__restrict__ force,
                              float *__restrict__ aos_out, float *__restrict__ xf_out, uint32_t n,
                              float sc)
 {
@@ -820,7 +852,7 @@ __global__ void k_pack_state(const float4 *__restrict__ posA, const float4 *__re
     dst[10] = f.y;
     dst[11] = f.z;
     dst[12] = rhoB[s];
-    dst[13] = pinvB[s].x;
+    dst[13] = velR[s].w;
     dst[14] = __uint_as_float(__float_as_uint(posB[s].w) & 0xFFFFu);
     if (xf_out)
         write_transform(xf_out + (size_t)s * 16, x.x, x.y, x.z, sc);
@@ -830,7 +862,7 @@ void launch_pack_state(hipStream_t st, const ParticleArrays &a, const float4 *fo
                        const StepParams &P, void *aos_out, float *xf_out)
 {
     hipLaunchKernelGGL(k_pack_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.posA, a.velA,
-                       a.posB, a.rhoB, a.pinvB, force, (float *)aos_out, xf_out, n, P.xf_scale);
+                       a.posB, a.rhoB, a.velR, force, (float *)aos_out, xf_out, n, P.xf_scale);
 }
 
 // ------------------------------------------------------------------ copy

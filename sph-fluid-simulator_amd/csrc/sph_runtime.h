@@ -39,7 +39,7 @@ struct DevBuf {
 
 // All device arrays of one particle set (see sph_kernels.h for the layout).
 struct Workspace {
-    DevBuf posA, velA, posB, velB, cellcA, rankA, cellcB, perm, rhoB, pinvB;
+    DevBuf posA, velA, posB, velB, cellcA, rankA, cellcB, perm, rhoB, posR, velR;
     DevBuf counts, cstart, partials;
     DevBuf force;
     uint32_t ncells_padded = 0;

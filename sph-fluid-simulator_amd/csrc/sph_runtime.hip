@@ -226,7 +226,8 @@ int Workspace::reserve_particles(uint32_t n)
         (rc = posB.reserve(sizeof(float4) * (size_t)n)) || (rc = velB.reserve(sizeof(float4) * (size_t)n)) ||
         (rc = cellcA.reserve(4ull * n)) || (rc = rankA.reserve(4ull * n)) ||
         (rc = cellcB.reserve(4ull * n)) || (rc = perm.reserve(4ull * n)) ||
-        (rc = rhoB.reserve(4ull * n)) || (rc = pinvB.reserve(8ull * n)))
+        (rc = rhoB.reserve(4ull * n)) || (rc = posR.reserve(sizeof(float4) * (size_t)n)) ||
+        (rc = velR.reserve(sizeof(float4) * (size_t)n)))
         return rc;
     return SPH_OK;
 }
@@ -262,7 +263,8 @@ ParticleArrays Workspace::particles() const
     a.cellcB = (uint32_t *)cellcB.ptr;
     a.perm = (uint32_t *)perm.ptr;
     a.rhoB = (float *)rhoB.ptr;
-    a.pinvB = (float2 *)pinvB.ptr;
+    a.posR = (float4 *)posR.ptr;
+    a.velR = (float4 *)velR.ptr;
     return a;
 }
 
@@ -278,7 +280,7 @@ CellArrays Workspace::cells() const
 
 void Workspace::release_all()
 {
-    for (DevBuf *b : {&posA, &velA, &posB, &velB, &cellcA, &rankA, &cellcB, &perm, &rhoB, &pinvB,
+    for (DevBuf *b : {&posA, &velA, &posB, &velB, &cellcA, &rankA, &cellcB, &perm, &rhoB, &posR, &velR,
                       &counts, &cstart, &partials, &force})
         b->release();
     ncells_padded = 0;
