@@ -163,6 +163,29 @@ int sph_hip_download_particles(sph_ctx *ctx, sph_particle_v1 *out,
 int sph_hip_set_graph(sph_ctx *ctx, int enable);
 uint32_t sph_hip_count(const sph_ctx *ctx);
 
+/* ---- batch of scenes ------------------------------------------------------
+ * Independent scenes (own settings, box, dt, grid, particle ids) advanced
+ * together: every pass is one launch over the particles of all scenes.  No
+ * counterpart in the reference (it steps one system); each scene's results
+ * are bit-identical to stepping it alone with sph_hip_*.  Every scene needs
+ * one upload before the first step.  Downloads return the scene's particles
+ * in device order with their ids. */
+typedef struct sph_batch sph_batch;
+
+int sph_batch_create(const sph_scene_desc_v1 *descs, uint32_t nscenes,
+                     sph_batch **out);
+int sph_batch_destroy(sph_batch *b);
+int sph_batch_upload(sph_batch *b, uint32_t scene, const float *pos,
+                     const float *vel);
+int sph_batch_step(sph_batch *b, int nsteps);
+int sph_batch_sync(sph_batch *b);
+int sph_batch_download(sph_batch *b, uint32_t scene, float *pos, float *vel,
+                       float *rho, float *pressure, uint32_t *ids);
+int sph_batch_set_graph(sph_batch *b, int enable);
+uint32_t sph_batch_scene_count(const sph_batch *b);
+int sph_batch_profile(sph_batch *b, int nsteps, double *ms, int max_k,
+                      int *nk, char *names_buf, size_t names_cap);
+
 /* ---- measurement ---------------------------------------------------------
  * Runs nsteps un-graphed steps with HIP events around every kernel on the
  * context's stream.  names: nul-separated list written into names_buf;

@@ -1,0 +1,540 @@
+// sph_batch.hip -- resident scenes: the single-scene context
+// (sph_hip_create ... destroy) and the multi-scene batch (sph_batch_*).
+//
+// A batch keeps any number of independent scenes in one set of HBM arrays:
+// scene s owns slots [p_begin, p_end) and grid cells [cell_base, cell_base +
+// nx ny nz), so one launch per pass advances every scene, and the global
+// counting sort never mixes scenes.  Each scene's grid depends only on its
+// own box and particles, and every per-particle result depends only on its
+// own scene, so a scene steps bit-identically alone or in any mix.  A step
+// is captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sph_batch.h"
+
+namespace sph {
+
+namespace {
+constexpr uint32_t kKnownFlags = SPH_SCENE_KEEP_FORCE;
+}
+
+int validate_desc(const sph_scene_desc_v1 &d)
+{
+    int rc;
+    if (d.n == 0)
+        return fail(SPH_E_INVALID, "scene: n must be > 0");
+    if (d.n > kMaxParticles)
+        return fail(SPH_E_LIMIT, "scene: %u particles exceed the 2^30 limit", d.n);
+    if ((d.flags & ~kKnownFlags) != 0u)
+        return fail(SPH_E_INVALID, "scene: unknown flags 0x%x", d.flags);
+    if ((rc = check_settings(&d.settings)))
+        return rc;
+    if (!std::isfinite(d.dt))
+        return fail(SPH_E_INVALID, "scene: dt must be finite");
+    if (!(d.cell_scale == 0.0f || (d.cell_scale >= 1.0f && std::isfinite(d.cell_scale))))
+        return fail(SPH_E_INVALID, "scene: cell_scale must be 0 or >= 1");
+    return SPH_OK;
+}
+
+// Grid of one scene from its box and particles: the finite walls, widened
+// to the particles; an open side gets head room of one particle extent (at
+// least 16 h).  Depends on nothing but the scene.
+void scene_grid(StepParams &P, const sph_scene_desc_v1 &d, const float *pos, uint32_t n)
+{
+    double pmin[3] = {INFINITY, INFINITY, INFINITY}, pmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double v = pos[3 * (size_t)i + k];
+            if (std::isfinite(v)) {
+                pmin[k] = std::min(pmin[k], v);
+                pmax[k] = std::max(pmax[k], v);
+            }
+        }
+    double lo[3], hi[3];
+    const double h = P.h;
+    for (int k = 0; k < 3; ++k) {
+        if (!(pmin[k] <= pmax[k])) {
+            pmin[k] = 0.0;
+            pmax[k] = 0.0;
+        }
+        double ext = std::max(pmax[k] - pmin[k], 16.0 * h);
+        double blo = d.box.lo[k], bhi = d.box.hi[k];
+        lo[k] = std::isfinite(blo) ? std::min(blo, pmin[k]) : pmin[k] - ext;
+        hi[k] = std::isfinite(bhi) ? std::max(bhi, pmax[k]) : pmax[k] + ext;
+    }
+    const double scale = d.cell_scale > 1.0f ? (double)d.cell_scale : 1.0;
+    const uint64_t max_cells = std::max<uint64_t>(64ull * n, 1ull << 16);
+    make_grid(P, lo, hi, scale, max_cells);
+}
+
+void Batch::drop_graph()
+{
+    if (exec)
+        (void)hipGraphExecDestroy(exec);
+    if (graph)
+        (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+}
+
+Batch::~Batch()
+{
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    drop_graph();
+    w.release_all();
+    if (stream)
+        (void)hipStreamDestroy(stream);
+}
+
+int Batch::init(const sph_scene_desc_v1 *descs, uint32_t k)
+{
+    int rc;
+    if (!descs || k == 0)
+        return fail(SPH_E_INVALID, "batch: no scenes");
+    uint64_t total = 0;
+    scenes.clear();
+    for (uint32_t s = 0; s < k; ++s) {
+        if ((rc = validate_desc(descs[s])))
+            return rc;
+        BatchScene b;
+        b.desc = descs[s];
+        b.P = make_params(descs[s].settings, descs[s].box, descs[s].dt);
+        b.P.p_begin = (uint32_t)total;
+        total += descs[s].n;
+        if (total > kMaxParticles)
+            return fail(SPH_E_LIMIT, "batch: %llu particles exceed the 2^30 limit", (unsigned long long)total);
+        b.P.p_end = (uint32_t)total;
+        keep_force = keep_force || (descs[s].flags & SPH_SCENE_KEEP_FORCE);
+        scenes.push_back(b);
+    }
+    total_n = (uint32_t)total;
+    if ((rc = ensure_device()))
+        return rc;
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        stream = nullptr;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    if ((rc = w.reserve_particles(total_n)) ||
+        (keep_force && (rc = w.force.reserve(sizeof(float4) * (size_t)total_n))))
+        return rc;
+    return SPH_OK;
+}
+
+int Batch::upload(uint32_t s, const float *pos, const float *vel)
+{
+    if (s >= scenes.size() || !pos)
+        return fail(SPH_E_INVALID, "upload: bad scene or NULL positions");
+    BatchScene &b = scenes[s];
+    const uint32_t n = b.desc.n, p0 = b.P.p_begin;
+    scene_grid(b.P, b.desc, pos, n);
+    std::vector<float4> p4(n), v4(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        p4[i] = make_float4(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2], 0.0f);
+        float vx = vel ? vel[3 * (size_t)i] : 0.0f, vy = vel ? vel[3 * (size_t)i + 1] : 0.0f;
+        float vz = vel ? vel[3 * (size_t)i + 2] : 0.0f;
+        uint32_t id = i;
+        float idf;
+        std::memcpy(&idf, &id, 4);
+        v4[i] = make_float4(vx, vy, vz, idf);
+    }
+    (void)hipStreamSynchronize(stream);
+    drop_graph();
+    ParticleArrays a = w.particles();
+    hipError_t e = hipMemcpyAsync(a.posA + p0, p4.data(), sizeof(float4) * (size_t)n, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(a.velA + p0, v4.data(), sizeof(float4) * (size_t)n, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "upload");
+    launch_fill_u32(stream, (uint32_t *)(a.rhoB + p0), 0u, n);
+    launch_fill_u32(stream, (uint32_t *)(a.posR + p0), 0u, 4ull * n);
+    launch_fill_u32(stream, (uint32_t *)(a.velR + p0), 0u, 4ull * n);
+    launch_fill_u32(stream, (uint32_t *)(a.posB + p0), 0u, 4ull * n);
+    if (keep_force)
+        launch_fill_u32(stream, (uint32_t *)w.force.ptr + 4ull * p0, 0u, 4ull * n);
+    e = hipStreamSynchronize(stream);
+    if (e == hipSuccess)
+        e = hipGetLastError();
+    if (e != hipSuccess)
+        return hip_fail(e, "upload");
+    b.uploaded = true;
+    layout_dirty = true;
+    keys_dirty = true;
+    return SPH_OK;
+}
+
+int Batch::prepare()
+{
+    int rc;
+    for (const BatchScene &b : scenes)
+        if (!b.uploaded)
+            return fail(SPH_E_STATE, "batch: every scene needs an upload before stepping");
+    if (layout_dirty) {
+        uint64_t base = 0;
+        std::vector<StepParams> table;
+        for (BatchScene &b : scenes) {
+            b.P.cell_base = (uint32_t)base;
+            base += (uint64_t)b.P.nx * b.P.ny * b.P.nz;
+            if (base > (1ull << 31))
+                return fail(SPH_E_LIMIT, "batch: %llu grid cells exceed the 2^31 limit", (unsigned long long)base);
+            table.push_back(b.P);
+        }
+        drop_graph();
+        if ((rc = w.reserve_cells(base)) || (rc = w.set_scenes(table, stream)))
+            return rc;
+        layout_dirty = false;
+        keys_dirty = true;
+    }
+    if (keys_dirty) {
+        CellArrays c = w.cells();
+        launch_fill_u32(stream, c.counts, 0u, c.ncells_padded);
+        launch_init_keys(stream, w.particles(), c, w.scene_set());
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return hip_fail(e, "keys");
+        keys_dirty = false;
+    }
+    return SPH_OK;
+}
+
+void Batch::enqueue(StepTimer *timer)
+{
+    enqueue_step(stream, w, keep_force ? (float4 *)w.force.ptr : nullptr, /*next_keys=*/true, timer);
+}
+
+int Batch::step(int nsteps)
+{
+    int rc;
+    if (nsteps < 0)
+        return fail(SPH_E_INVALID, "step: negative step count");
+    if ((rc = prepare()))
+        return rc;
+    if (use_graph) {
+        if (!exec) {
+            hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess)
+                return hip_fail(e, "hipStreamBeginCapture");
+            enqueue(nullptr);
+            e = hipStreamEndCapture(stream, &graph);
+            if (e == hipSuccess)
+                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                drop_graph();
+                return hip_fail(e, "graph capture");
+            }
+        }
+        for (int k = 0; k < nsteps; ++k) {
+            hipError_t e = hipGraphLaunch(exec, stream);
+            if (e != hipSuccess)
+                return hip_fail(e, "hipGraphLaunch");
+        }
+    } else {
+        for (int k = 0; k < nsteps; ++k)
+            enqueue(nullptr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return hip_fail(e, "step launch");
+    }
+    return SPH_OK;
+}
+
+int Batch::sync()
+{
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "hipStreamSynchronize");
+    return SPH_OK;
+}
+
+int Batch::download(uint32_t s, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids)
+{
+    if (s >= scenes.size())
+        return fail(SPH_E_INVALID, "download: bad scene");
+    if (!scenes[s].uploaded)
+        return fail(SPH_E_STATE, "download: scene not uploaded");
+    const uint32_t n = scenes[s].desc.n, p0 = scenes[s].P.p_begin;
+    ParticleArrays a = w.particles();
+    std::vector<float4> p4, v4, r4;
+    std::vector<float> r;
+    hipError_t e = hipSuccess;
+    if (pos) {
+        p4.resize(n);
+        e = hipMemcpyAsync(p4.data(), a.posA + p0, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, stream);
+    }
+    if (e == hipSuccess && (vel || ids)) {
+        v4.resize(n);
+        e = hipMemcpyAsync(v4.data(), a.velA + p0, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, stream);
+    }
+    if (e == hipSuccess && rho) {
+        r.resize(n);
+        e = hipMemcpyAsync(r.data(), a.rhoB + p0, 4ull * n, hipMemcpyDeviceToHost, stream);
+    }
+    if (e == hipSuccess && pressure) {
+        r4.resize(n);
+        e = hipMemcpyAsync(r4.data(), a.velR + p0, sizeof(float4) * (size_t)n, hipMemcpyDeviceToHost, stream);
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "download");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (pos) {
+            pos[3 * (size_t)i] = p4[i].x;
+            pos[3 * (size_t)i + 1] = p4[i].y;
+            pos[3 * (size_t)i + 2] = p4[i].z;
+        }
+        if (vel) {
+            vel[3 * (size_t)i] = v4[i].x;
+            vel[3 * (size_t)i + 1] = v4[i].y;
+            vel[3 * (size_t)i + 2] = v4[i].z;
+        }
+        if (ids)
+            std::memcpy(&ids[i], &v4[i].w, 4);
+        if (rho)
+            rho[i] = r[i];
+        if (pressure)
+            pressure[i] = r4[i].w;
+    }
+    return SPH_OK;
+}
+
+int Batch::download_particles(uint32_t s, sph_particle_v1 *out, float *transforms)
+{
+    int rc;
+    if (s >= scenes.size() || !out)
+        return fail(SPH_E_INVALID, "download_particles: bad scene or NULL output");
+    if (!scenes[s].uploaded)
+        return fail(SPH_E_STATE, "download_particles: scene not uploaded");
+    const uint32_t n = scenes[s].desc.n, p0 = scenes[s].P.p_begin;
+    if ((rc = aos.reserve(sizeof(sph_particle_v1) * (size_t)n)) || (transforms && (rc = xf.reserve(64ull * n))))
+        return rc;
+    launch_pack_state(stream, w.particles(), keep_force ? (const float4 *)w.force.ptr : nullptr, p0, n,
+                      scenes[s].P.xf_scale, aos.ptr, transforms ? (float *)xf.ptr : nullptr);
+    hipError_t e = hipMemcpyAsync(out, aos.ptr, sizeof(sph_particle_v1) * (size_t)n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess && transforms)
+        e = hipMemcpyAsync(transforms, xf.ptr, 64ull * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "download_particles");
+    return SPH_OK;
+}
+
+int Batch::profile(int nsteps, double *ms, int max_k, int *nk, char *names_buf, size_t names_cap)
+{
+    int rc;
+    if (nsteps <= 0 || !ms || !nk || max_k <= 0)
+        return fail(SPH_E_INVALID, "profile: bad arguments");
+    if ((rc = prepare()))
+        return rc;
+    std::vector<double> acc;
+    std::vector<const char *> names;
+    for (int s = 0; s < nsteps; ++s) {
+        StepTimer t;
+        enqueue(&t);
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess)
+            return hip_fail(e, "profile step");
+        if (acc.empty()) {
+            acc.assign(t.events.size() - 1, 0.0);
+            names.assign(t.names.begin() + 1, t.names.end());
+        }
+        for (size_t k = 1; k < t.events.size(); ++k) {
+            float dt = 0.0f;
+            (void)hipEventElapsedTime(&dt, t.events[k - 1], t.events[k]);
+            acc[k - 1] += dt;
+        }
+    }
+    int k_out = (int)std::min<size_t>(acc.size(), (size_t)max_k);
+    size_t pos = 0;
+    for (int k = 0; k < k_out; ++k) {
+        ms[k] = acc[k] / nsteps;
+        if (names_buf) {
+            size_t len = std::strlen(names[k]);
+            if (pos + len + 1 <= names_cap) {
+                std::memcpy(names_buf + pos, names[k], len + 1);
+                pos += len + 1;
+            }
+        }
+    }
+    *nk = k_out;
+    return SPH_OK;
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+struct sph_ctx {
+    Batch b;
+};
+
+struct sph_batch {
+    Batch b;
+};
+
+namespace {
+
+template <class T>
+int create_object(const sph_scene_desc_v1 *descs, uint32_t k, T **out)
+{
+    if (!out)
+        return fail(SPH_E_INVALID, "create: NULL output");
+    *out = nullptr;
+    T *o = new (std::nothrow) T();
+    if (!o)
+        return fail(SPH_E_NOMEM, "create: host allocation failed");
+    int rc = o->b.init(descs, k);
+    if (rc) {
+        delete o;
+        return rc;
+    }
+    *out = o;
+    return SPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- single scene
+
+int sph_hip_create(const sph_scene_desc_v1 *desc, sph_ctx **out)
+{
+    if (!desc)
+        return fail(SPH_E_INVALID, "create: NULL argument");
+    return create_object(desc, 1u, out);
+}
+
+int sph_hip_destroy(sph_ctx *c)
+{
+    delete c;
+    return SPH_OK;
+}
+
+int sph_hip_upload(sph_ctx *c, const float *pos, const float *vel)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "upload: NULL context");
+    return c->b.upload(0, pos, vel);
+}
+
+int sph_hip_step(sph_ctx *c, int nsteps)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "step: NULL context");
+    return c->b.step(nsteps);
+}
+
+int sph_hip_sync(sph_ctx *c)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "sync: NULL context");
+    return c->b.sync();
+}
+
+int sph_hip_download(sph_ctx *c, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "download: NULL context");
+    return c->b.download(0, pos, vel, rho, pressure, ids);
+}
+
+int sph_hip_download_particles(sph_ctx *c, sph_particle_v1 *out, float *transforms)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "download_particles: NULL context");
+    return c->b.download_particles(0, out, transforms);
+}
+
+int sph_hip_set_graph(sph_ctx *c, int enable)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "set_graph: NULL context");
+    c->b.use_graph = enable != 0;
+    if (!c->b.use_graph)
+        c->b.drop_graph();
+    return SPH_OK;
+}
+
+uint32_t sph_hip_count(const sph_ctx *c) { return c ? c->b.total_n : 0u; }
+
+int sph_hip_profile(sph_ctx *c, int nsteps, double *ms, int max_k, int *nk, char *names_buf,
+                    size_t names_cap)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "profile: NULL context");
+    return c->b.profile(nsteps, ms, max_k, nk, names_buf, names_cap);
+}
+
+// ---- batch of scenes
+
+int sph_batch_create(const sph_scene_desc_v1 *descs, uint32_t nscenes, sph_batch **out)
+{
+    return create_object(descs, nscenes, out);
+}
+
+int sph_batch_destroy(sph_batch *b)
+{
+    delete b;
+    return SPH_OK;
+}
+
+int sph_batch_upload(sph_batch *b, uint32_t scene, const float *pos, const float *vel)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch upload: NULL batch");
+    return b->b.upload(scene, pos, vel);
+}
+
+int sph_batch_step(sph_batch *b, int nsteps)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch step: NULL batch");
+    return b->b.step(nsteps);
+}
+
+int sph_batch_sync(sph_batch *b)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch sync: NULL batch");
+    return b->b.sync();
+}
+
+int sph_batch_download(sph_batch *b, uint32_t scene, float *pos, float *vel, float *rho,
+                       float *pressure, uint32_t *ids)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch download: NULL batch");
+    return b->b.download(scene, pos, vel, rho, pressure, ids);
+}
+
+int sph_batch_set_graph(sph_batch *b, int enable)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch set_graph: NULL batch");
+    b->b.use_graph = enable != 0;
+    if (!b->b.use_graph)
+        b->b.drop_graph();
+    return SPH_OK;
+}
+
+uint32_t sph_batch_scene_count(const sph_batch *b) { return b ? (uint32_t)b->b.scenes.size() : 0u; }
+
+int sph_batch_profile(sph_batch *b, int nsteps, double *ms, int max_k, int *nk, char *names_buf,
+                      size_t names_cap)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch profile: NULL batch");
+    return b->b.profile(nsteps, ms, max_k, nk, names_buf, names_cap);
+}
+
+}  // extern "C"

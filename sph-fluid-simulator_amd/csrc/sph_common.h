@@ -70,6 +70,8 @@ struct StepParams {
     double grid_inv;  // 1 / H
     int nx, ny, nz;
     uint32_t cell_base;  // first linear cell of this scene
+    // slots [p_begin, p_end) of the particle arrays belong to this scene
+    uint32_t p_begin, p_end;
 };
 
 SPH_HD inline uint32_t hash16(uint32_t cx, uint32_t cy, uint32_t cz)

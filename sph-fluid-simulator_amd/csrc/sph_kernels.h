@@ -15,6 +15,12 @@
 //   posR / velR are what the force pass gathers: one record for the
 //   candidate filter and one more per neighbour.
 //   cells: counts[ncells] (zero between steps), cstart[ncells + 1]
+//
+// Several independent scenes can share the arrays ("batch"): scene s owns
+// the slot range [p_begin, p_end) of both orders and the cell range
+// [cell_base, cell_base + nx ny nz).  Per-particle kernels run one 64-slot
+// wave per task (scene, first slot), so every wave reads one scene's
+// constants; a single scene is a batch of one.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,14 +32,13 @@ namespace sph {
 
 constexpr int kBlock = 256;           // threads per block, particle kernels
 constexpr int kStencilBlock = 128;    // threads per block, density / force
-#ifndef SPH_STAGE_CAP
-#define SPH_STAGE_CAP 0
-#endif
-constexpr uint32_t kStageCap = SPH_STAGE_CAP;  // positions staged in LDS per block (0: off)
 constexpr int kForceChunk = 16;       // candidates filtered per bitmask in the force pass
 constexpr int kScanItems = 16;        // cells per thread in the scan
 constexpr uint32_t kScanChunk = kBlock * kScanItems;  // cells per scan block
-constexpr uint32_t kStableCellMax = 512;  // larger cells keep arrival order
+constexpr uint32_t kSmallCell = 64;        // cells up to this size: stable rank by counting
+constexpr uint32_t kBigCellSortMax = 16384;  // larger cells keep arrival order
+constexpr int kBigSortBlock = 1024;        // threads of the big-cell sort
+constexpr int kBigSortBlocks = 64;         // blocks walking the big-cell list
 
 struct ParticleArrays {
     float4 *posA, *velA, *posB, *velB;
@@ -46,7 +51,21 @@ struct CellArrays {
     uint32_t *counts;  // ncells_padded entries, zero between steps
     uint32_t *cstart;  // ncells_padded + 1 entries
     uint32_t *partials;
+    uint32_t *big;     // [0] = number of big cells, then their indices (may be null)
+    uint32_t big_cap;  // capacity of the index list
     uint32_t ncells_padded;  // multiple of kScanChunk
+};
+
+struct WaveTask {
+    uint32_t scene;
+    uint32_t first;  // first slot of the wave
+};
+
+// Device view of the scenes of a batch.
+struct SceneSet {
+    const StepParams *params;  // one per scene
+    const WaveTask *tasks;     // one per 64-slot wave
+    uint32_t nwaves;
 };
 
 // Debug outputs of the density pass: per-particle neighbour count (with
@@ -63,28 +82,28 @@ inline uint32_t blocks_for(uint32_t n, int block = kBlock) { return (n + block -
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t value, size_t n);
 void launch_unpack_aos(hipStream_t st, const void *aos, uint32_t n, float4 *posA, float4 *velA);
 void launch_bbox(hipStream_t st, const float4 *pos, uint32_t n, int32_t *bbox6);
-void launch_init_keys(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                      const StepParams &P);
-void launch_scan(hipStream_t st, const CellArrays &c, uint32_t n_total);
-void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                    const StepParams &P);
-void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                    const StepParams &P);
-void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                    const StepParams &P, const NeighborDump *dump);
+void launch_init_keys(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S);
+// Scan of the cell counts; also lists the cells above kSmallCell in c.big.
+void launch_scan(hipStream_t st, const CellArrays &c);
+void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S);
+// Sorts the A indices of every listed big cell (deterministic in-cell order).
+void launch_sort_big_cells(hipStream_t st, const ParticleArrays &a, const CellArrays &c);
+void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S);
+void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
+                    const NeighborDump *dump);
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                            uint32_t n, const StepParams &P, float4 *force_out, bool next_keys);
-// Drop-in output: order slots by the pre-step 16-bit bucket (ties by id),
-// and write the reference AoS layout plus transforms.
+                            const SceneSet &S, float4 *force_out, bool next_keys);
+// Drop-in output (single scene): order slots by the pre-step 16-bit bucket
+// (ties by id), and write the reference AoS layout plus transforms.
 void launch_h16_rank(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n);
 void launch_h16_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n,
                         uint32_t *perm16);
 void launch_pack_out(hipStream_t st, const ParticleArrays &a, const CellArrays &c16,
                      const uint32_t *perm16, const float4 *force, const void *aos_in, uint32_t n,
-                     const StepParams &P, void *aos_out, float *xf_out);
-// Resident download: AoS of the current A state (device order).
-void launch_pack_state(hipStream_t st, const ParticleArrays &a, const float4 *force, uint32_t n,
-                       const StepParams &P, void *aos_out, float *xf_out);
+                     float xf_scale, void *aos_out, float *xf_out);
+// Resident download: AoS of slots [first, first + n) of the A state.
+void launch_pack_state(hipStream_t st, const ParticleArrays &a, const float4 *force, uint32_t first,
+                       uint32_t n, float xf_scale, void *aos_out, float *xf_out);
 void launch_copy(hipStream_t st, const uint4 *src, uint4 *dst, size_t n16);
 
 }  // namespace sph

@@ -1,8 +1,10 @@
 // sph_kernels.hip -- hand-written CDNA4 (gfx950) kernels of one SPH step.
 //
-// One step (single scene) is, in launch order:
-//   scan          exclusive scan of per-cell counts -> cstart, counts := 0
+// One step is, in launch order:
+//   scan          exclusive scan of per-cell counts -> cstart, counts := 0,
+//                 and the list of cells holding more than kSmallCell
 //   scatter       perm[cstart[cell] + rank] = A index
+//   sort_big      sorts the A indices inside each listed big cell
 //   reorder       stable cell order (ties by A index), gather A -> B
 //   density       poly6 density + pressure        (src/sph.cpp:28-76)
 //   force_integrate
@@ -36,7 +38,29 @@ __device__ inline uint32_t xcd_block()
     return xcd * per + (xcd < rem ? xcd : rem) + k;
 }
 
-__device__ inline uint32_t particle_index() { return xcd_block() * blockDim.x + threadIdx.x; }
+// The wave task of the calling wave: its scene (wave-uniform, so the
+// scene's constants are scalar loads) and its slot.  Returns false for
+// waves past the task list and lanes past the scene's last slot.
+struct Lane {
+    const StepParams *P;
+    uint32_t slot;
+};
+
+template <int BLOCK>
+__device__ inline bool lane_of(const SceneSet &S, Lane &out)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
+    if (w >= S.nwaves)
+        return false;
+    const WaveTask t = S.tasks[w];
+    const uint32_t scene = __builtin_amdgcn_readfirstlane(t.scene);
+    const uint32_t first = __builtin_amdgcn_readfirstlane(t.first);
+    out.P = S.params + scene;
+    out.slot = first + (threadIdx.x & 63u);
+    return out.slot < out.P->p_end;
+}
+
+inline uint32_t blocks_for_waves(uint32_t nwaves, int block) { return (nwaves + block / 64 - 1) / (block / 64); }
 
 // ---------------------------------------------------------------- utilities
 
@@ -131,13 +155,16 @@ void launch_bbox(hipStream_t st, const float4 *pos, uint32_t n, int32_t *bbox6)
 }
 
 // Cell key, bucket and in-cell arrival rank of the positions in posA.
-__global__ void k_init_keys(float4 *__restrict__ posA, uint32_t *__restrict__ cellcA,
-                            uint32_t *__restrict__ rankA, uint32_t *__restrict__ counts, uint32_t n,
-                            StepParams P)
+__global__ __launch_bounds__(kBlock) void k_init_keys(float4 *__restrict__ posA,
+                                                     uint32_t *__restrict__ cellcA,
+                                                     uint32_t *__restrict__ rankA,
+                                                     uint32_t *__restrict__ counts, SceneSet S)
 {
-    uint32_t i = particle_index();
-    if (i >= n)
+    Lane L;
+    if (!lane_of<kBlock>(S, L))
         return;
+    const StepParams &P = *L.P;
+    const uint32_t i = L.slot;
     float4 p = posA[i];
     Classified c = classify(p.x, p.y, p.z, P);
     posA[i].w = __uint_as_float(c.h16);
@@ -145,17 +172,17 @@ __global__ void k_init_keys(float4 *__restrict__ posA, uint32_t *__restrict__ ce
     rankA[i] = atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
 }
 
-void launch_init_keys(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                      const StepParams &P)
+void launch_init_keys(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S)
 {
-    hipLaunchKernelGGL(k_init_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.posA, a.cellcA,
-                       a.rankA, c.counts, n, P);
+    hipLaunchKernelGGL(k_init_keys, dim3(blocks_for_waves(S.nwaves, kBlock)), dim3(kBlock), 0, st,
+                       a.posA, a.cellcA, a.rankA, c.counts, S);
 }
 
 // ------------------------------------------------------------- cell scan
 // Three passes over the (padded) cell array: per-block totals, a one-block
 // scan of the totals, then the block-local exclusive scan plus offset.  The
-// last pass also clears the counts for the next step's atomics.
+// last pass also clears the counts for the next step's atomics and appends
+// every cell above kSmallCell to the big-cell list.
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t x)
 {
@@ -207,9 +234,9 @@ __global__ void k_scan_reduce(const uint32_t *__restrict__ counts, uint32_t *__r
 }
 
 // One block: exclusive scan of nparts block totals in place; cstart_end
-// receives the grand total.
+// receives the grand total.  Also clears the big-cell count.
 __global__ void k_scan_partials(uint32_t *__restrict__ partials, uint32_t nparts,
-                                uint32_t *__restrict__ cstart_end)
+                                uint32_t *__restrict__ cstart_end, uint32_t *__restrict__ big)
 {
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nparts; base += kBlock) {
@@ -221,12 +248,15 @@ __global__ void k_scan_partials(uint32_t *__restrict__ partials, uint32_t nparts
             partials[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         *cstart_end = carry;
+        if (big)
+            big[0] = 0u;
+    }
 }
 
 __global__ void k_scan_final(uint32_t *__restrict__ counts, const uint32_t *__restrict__ partials,
-                             uint32_t *__restrict__ cstart)
+                             uint32_t *__restrict__ cstart, uint32_t *__restrict__ big, uint32_t big_cap)
 {
     size_t base = (size_t)blockIdx.x * kScanChunk + (size_t)threadIdx.x * kScanItems;
     uint4 *src = (uint4 *)(counts + base);
@@ -260,54 +290,117 @@ __global__ void k_scan_final(uint32_t *__restrict__ counts, const uint32_t *__re
         dst[k] = q;
         src[k] = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (big) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            if (v[k] > kSmallCell) {
+                const uint32_t at = atomicAdd(&big[0], 1u);
+                if (at < big_cap)
+                    big[1 + at] = (uint32_t)(base + k);
+            }
+        }
+    }
 }
 
-void launch_scan(hipStream_t st, const CellArrays &c, uint32_t /*n_total*/)
+void launch_scan(hipStream_t st, const CellArrays &c)
 {
     uint32_t nparts = c.ncells_padded / kScanChunk;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kBlock), 0, st, c.counts, c.partials);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, st, c.partials, nparts,
-                       c.cstart + c.ncells_padded);
+                       c.cstart + c.ncells_padded, c.big);
     hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, st, c.counts, c.partials,
-                       c.cstart);
+                       c.cstart, c.big, c.big_cap);
 }
 
 // ---------------------------------------------------------- scatter/reorder
 
-__global__ void k_scatter(const uint32_t *__restrict__ cellcA, const uint32_t *__restrict__ rankA,
-                          const uint32_t *__restrict__ cstart, uint32_t *__restrict__ perm,
-                          uint32_t n, StepParams P)
+__global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t *__restrict__ cellcA,
+                                                   const uint32_t *__restrict__ rankA,
+                                                   const uint32_t *__restrict__ cstart,
+                                                   uint32_t *__restrict__ perm, SceneSet S)
 {
-    uint32_t i = particle_index();
-    if (i >= n)
+    Lane L;
+    if (!lane_of<kBlock>(S, L))
         return;
-    perm[cstart[linear_cell(P, cellcA[i])] + rankA[i]] = i;
+    const uint32_t i = L.slot;
+    perm[cstart[linear_cell(*L.P, cellcA[i])] + rankA[i]] = i;
 }
 
-void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                    const StepParams &P)
+void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S)
 {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.cellcA, a.rankA,
-                       c.cstart, a.perm, n, P);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for_waves(S.nwaves, kBlock)), dim3(kBlock), 0, st,
+                       a.cellcA, a.rankA, c.cstart, a.perm, S);
 }
 
-// Slot s holds A index i in arrival order; its stable destination is the
-// cell start plus the number of smaller A indices in the same cell.
-__global__ void k_reorder(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ cellcA,
-                          const uint32_t *__restrict__ cstart, const float4 *__restrict__ posA,
-                          const float4 *__restrict__ velA, float4 *__restrict__ posB,
-                          float4 *__restrict__ velB, uint32_t *__restrict__ cellcB, uint32_t n,
-                          StepParams P)
+// Big cells: one block sorts the A indices of one listed cell in LDS
+// (bitonic, ascending), so slot order inside the cell is the stable order.
+// Blocks walk the list with a stride; every block exits when the list ends.
+__global__ __launch_bounds__(kBigSortBlock) void k_sort_big_cells(
+    uint32_t *__restrict__ perm, const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ big,
+    uint32_t big_cap)
 {
-    uint32_t s = particle_index();
-    if (s >= n)
+    __shared__ uint32_t sh[kBigCellSortMax];
+    const uint32_t nbig = min(big[0], big_cap);
+    for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
+        const uint32_t cell = big[1 + e];
+        const uint32_t b = cstart[cell], len = cstart[cell + 1] - b;
+        if (len > kBigCellSortMax)
+            continue;  // uniform: keeps arrival order
+        uint32_t m = 1;
+        while (m < len)
+            m <<= 1;
+        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x)
+            sh[k] = k < len ? perm[b + k] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t size = 2; size <= m; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+                    const uint32_t partner = k ^ stride;
+                    if (partner > k) {
+                        const bool up = (k & size) == 0u;
+                        const uint32_t x = sh[k], y = sh[partner];
+                        if ((x > y) == up) {
+                            sh[k] = y;
+                            sh[partner] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t k = threadIdx.x; k < len; k += blockDim.x)
+            perm[b + k] = sh[k];
+        __syncthreads();
+    }
+}
+
+void launch_sort_big_cells(hipStream_t st, const ParticleArrays &a, const CellArrays &c)
+{
+    if (!c.big)
         return;
-    uint32_t i = perm[s];
-    uint32_t c = cellcA[i];
-    uint32_t lin = linear_cell(P, c);
-    uint32_t b = cstart[lin], e = cstart[lin + 1];
+    hipLaunchKernelGGL(k_sort_big_cells, dim3(kBigSortBlocks), dim3(kBigSortBlock), 0, st, a.perm,
+                       c.cstart, c.big, c.big_cap);
+}
+
+// Slot s holds A index i; its stable destination is the cell start plus the
+// number of smaller A indices in the same cell.  Cells above kSmallCell
+// were sorted by k_sort_big_cells, so their destination is s itself.
+__global__ __launch_bounds__(kBlock) void k_reorder(
+    const uint32_t *__restrict__ perm, const uint32_t *__restrict__ cellcA,
+    const uint32_t *__restrict__ cstart, const float4 *__restrict__ posA,
+    const float4 *__restrict__ velA, float4 *__restrict__ posB, float4 *__restrict__ velB,
+    uint32_t *__restrict__ cellcB, SceneSet S)
+{
+    Lane L;
+    if (!lane_of<kBlock>(S, L))
+        return;
+    const uint32_t s = L.slot;
+    const uint32_t i = perm[s];
+    const uint32_t c = cellcA[i];
+    const uint32_t lin = linear_cell(*L.P, c);
+    const uint32_t b = cstart[lin], e = cstart[lin + 1];
     uint32_t d;
-    if (e - b <= kStableCellMax) {
+    if (e - b <= kSmallCell) {
         uint32_t r = 0;
         for (uint32_t t = b; t < e; ++t)
             r += (perm[t] < i) ? 1u : 0u;
@@ -320,108 +413,13 @@ __global__ void k_reorder(const uint32_t *__restrict__ perm, const uint32_t *__r
     cellcB[d] = c;
 }
 
-void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                    const StepParams &P)
+void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S)
 {
-    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.perm, a.cellcA,
-                       c.cstart, a.posA, a.velA, a.posB, a.velB, a.cellcB, n, P);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks_for_waves(S.nwaves, kBlock)), dim3(kBlock), 0, st,
+                       a.perm, a.cellcA, c.cstart, a.posA, a.velA, a.posB, a.velB, a.cellcB, S);
 }
 
 // ------------------------------------------------------------ stencil walk
-// The stencil kernels run blocks of kStencilBlock consecutive particles of
-// the cell-ordered arrays.  Their cells lie in one linear range [L0, L1], so
-// for each of the 9 (dy, dz) rows the candidates of every particle of the
-// block lie in one contiguous "band" of the cell-ordered arrays: cells
-// [L0 - 1 + off, L1 + 1 + off], off = nx (dy + ny dz).  The block copies the
-// 9 bands of positions into LDS once (coalesced) and every lane then walks
-// its own three-cell x-run of each row out of LDS.  A block whose bands
-// exceed kStageCap walks global memory instead (same order, same results).
-struct StageInfo {
-    uint32_t src[9];   // first cell-ordered index of band r
-    uint32_t dst[10];  // LDS offset of band r; dst[9] = total staged
-};
-
-// Fills `info` and, when the bands fit, `sh`.  Every thread of the block
-// must call it (it synchronises); returns whether the bands are staged.
-__device__ inline bool stage_bands(const float4 *__restrict__ pos, const uint32_t *__restrict__ cellc,
-                                   const uint32_t *__restrict__ cstart, uint32_t first, uint32_t last,
-                                   const StepParams &P, StageInfo &info, float4 *__restrict__ sh)
-{
-    if (threadIdx.x < 9) {
-        const int r = threadIdx.x;
-        const int dz = r / 3 - 1, dy = r % 3 - 1;
-        const int64_t L0 = linear_cell(P, cellc[first]);
-        const int64_t L1 = linear_cell(P, cellc[last]);
-        const int64_t off = (int64_t)P.nx * ((int64_t)dy + (int64_t)P.ny * dz);
-        const int64_t lo = P.cell_base;
-        const int64_t hi = (int64_t)P.cell_base + (int64_t)P.nx * P.ny * P.nz;
-        int64_t a = L0 - 1 + off, b = L1 + 2 + off;
-        a = a < lo ? lo : (a > hi ? hi : a);
-        b = b < lo ? lo : (b > hi ? hi : b);
-        if (b < a)
-            b = a;
-        const uint32_t ca = cstart[a];
-        info.src[r] = ca;
-        info.dst[r + 1] = cstart[b] - ca;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        info.dst[0] = 0;
-        for (int r = 0; r < 9; ++r)
-            info.dst[r + 1] += info.dst[r];
-    }
-    __syncthreads();
-    const bool staged = info.dst[9] <= kStageCap;
-    if (staged) {
-        for (int r = 0; r < 9; ++r) {
-            const uint32_t src = info.src[r], d0 = info.dst[r], len = info.dst[r + 1] - d0;
-            for (uint32_t k = threadIdx.x; k < len; k += blockDim.x)
-                sh[d0 + k] = pos[src + k];
-        }
-    }
-    __syncthreads();
-    return staged;
-}
-
-// Calls body(j, q) for every candidate j != s in the 3x3x3 grid-cell block
-// around the particle (q = its position record), reading positions from LDS
-// when STAGED.  The x-run of three cells in each of the 9 rows is one
-// contiguous range [b, e) of the cell-ordered arrays, inside band r.
-template <bool STAGED, class Body>
-__device__ inline void walk_candidates(const float4 *__restrict__ pos, const float4 *__restrict__ sh,
-                                       const StageInfo &info, const uint32_t *__restrict__ cstart,
-                                       uint32_t s, uint32_t cc, const StepParams &P, Body body)
-{
-    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc);
-    const int x0 = gx > 0 ? gx - 1 : 0;
-    const int x1 = gx + 1 < P.nx ? gx + 1 : P.nx - 1;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = gz + dz;
-        if (z < 0 || z >= P.nz)
-            continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = gy + dy;
-            if (y < 0 || y >= P.ny)
-                continue;
-            const uint32_t row = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
-            const uint32_t b = cstart[row + (uint32_t)x0];
-            const uint32_t e = cstart[row + (uint32_t)x1 + 1u];
-            const int r = (dz + 1) * 3 + (dy + 1);
-            const uint32_t shift = STAGED ? info.dst[r] - info.src[r] : 0u;
-            uint32_t j = b;
-            for (; j + 1u < e; j += 2u) {  // two candidates in flight
-                const float4 q0 = STAGED ? sh[j + shift] : pos[j];
-                const float4 q1 = STAGED ? sh[j + 1u + shift] : pos[j + 1u];
-                if (j != s)
-                    body(j, q0);
-                if (j + 1u != s)
-                    body(j + 1u, q1);
-            }
-            if (j < e && j != s)
-                body(j, STAGED ? sh[j + shift] : pos[j]);
-        }
-    }
-}
 
 // fl(fl(fl(dx*dx) + fl(dy*dy)) + fl(dz*dz)), dx = x_j - x_i: glm::length2.
 __device__ inline float dist2_ref(float4 pi, float4 q, float &dx, float &dy, float &dz)
@@ -458,42 +456,61 @@ __device__ inline RefCell ref_cell(float4 p, const StepParams &P)
     return RefCell{trunc_to_int(p.x / P.h), trunc_to_int(p.y / P.h), trunc_to_int(p.z / P.h)};
 }
 
+// The 9 rows of the 3x3x3 grid-cell stencil around packed cell cc: for row
+// (dy, dz) the x-run of three cells is one contiguous range [b, e) of the
+// cell-ordered arrays.  Calls row(b, e) for each row inside the grid.
+template <class Row>
+__device__ inline void for_each_row(const uint32_t *__restrict__ cstart, uint32_t cc,
+                                    const StepParams &P, Row row)
+{
+    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc);
+    const int x0 = gx > 0 ? gx - 1 : 0;
+    const int x1 = gx + 1 < P.nx ? gx + 1 : P.nx - 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = gz + dz;
+        if (z < 0 || z >= P.nz)
+            continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = gy + dy;
+            if (y < 0 || y >= P.ny)
+                continue;
+            const uint32_t r0 = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
+            row(cstart[r0 + (uint32_t)x0], cstart[r0 + (uint32_t)x1 + 1u]);
+        }
+    }
+}
+
 // --------------------------------------------------------------- density
 // MODE 0: density + pressure.  MODE 1: also the neighbour count per
 // particle.  MODE 2: also the neighbour ids (multiplicity-repeated).
+// Writes rho, posR = (x, 0.5 / rho) and velR = (v, p) for the force pass.
 template <int MODE>
 __global__ __launch_bounds__(kStencilBlock) void k_density(
     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
     const uint32_t *__restrict__ cellc, const uint32_t *__restrict__ cstart,
-    float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, uint32_t n,
-    StepParams P, NeighborDump dump)
+    float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, SceneSet S,
+    NeighborDump dump)
 {
-    __shared__ float4 sh[kStageCap > 0 ? kStageCap : 1];
-    __shared__ StageInfo info;
-    const uint32_t first = xcd_block() * kStencilBlock;
-    if (first >= n)
-        return;  // whole block
-    const uint32_t last = min(first + (uint32_t)kStencilBlock, n) - 1u;
-    bool staged = false;
-    if constexpr (kStageCap > 0)
-        staged = stage_bands(pos, cellc, cstart, first, last, P, info, sh);
-    const uint32_t s = first + threadIdx.x;
-    if (s >= n)
+    Lane L;
+    if (!lane_of<kStencilBlock>(S, L))
         return;
+    const StepParams &P = *L.P;
+    const uint32_t s = L.slot;
     const float4 pi = pos[s];
     const uint32_t cc = cellc[s];
     const bool slow = (cc & kSlowBit) != 0u;
     const RefCell ci = ref_cell(pi, P);
+    const float h2 = P.h2, mp6 = P.mp6;
     float acc = 0.0f;
     uint32_t cnt = 0;
     uint64_t w = (MODE == 2) ? dump.off[s] : 0;
     auto body = [&](uint32_t j, float4 q) {
         float dx, dy, dz;
         float d2 = dist2_ref(pi, q, dx, dy, dz);
-        if (d2 < P.h2) {
+        if (d2 < h2) {
             int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
-            float t = P.h2 - d2;
-            float term = P.mp6 * ((t * t) * t);
+            float t = h2 - d2;
+            float term = mp6 * ((t * t) * t);
             acc += (float)m * term;
             if (MODE >= 1)
                 cnt += (uint32_t)m;
@@ -504,10 +521,19 @@ __global__ __launch_bounds__(kStencilBlock) void k_density(
             }
         }
     };
-    if (staged)
-        walk_candidates<true>(pos, sh, info, cstart, s, cc, P, body);
-    else
-        walk_candidates<false>(pos, sh, info, cstart, s, cc, P, body);
+    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+        uint32_t j = b;
+        for (; j + 1u < e; j += 2u) {  // two candidates in flight
+            const float4 q0 = pos[j];
+            const float4 q1 = pos[j + 1u];
+            if (j != s)
+                body(j, q0);
+            if (j + 1u != s)
+                body(j + 1u, q1);
+        }
+        if (j < e && j != s)
+            body(j, pos[j]);
+    });
     const float r = acc + P.self_dens;
     const float p = P.gas * (r - P.rho0);
     rho[s] = r;
@@ -518,20 +544,20 @@ __global__ __launch_bounds__(kStencilBlock) void k_density(
         dump.count[s] = cnt;
 }
 
-void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, uint32_t n,
-                    const StepParams &P, const NeighborDump *dump)
+void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
+                    const NeighborDump *dump)
 {
     NeighborDump d = dump ? *dump : NeighborDump{nullptr, nullptr, nullptr};
-    dim3 g(blocks_for(n, kStencilBlock)), b(kStencilBlock);
+    dim3 g(blocks_for_waves(S.nwaves, kStencilBlock)), b(kStencilBlock);
     if (!dump)
         hipLaunchKernelGGL(k_density<0>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.posR, a.velR, n, P, d);
+                           a.posR, a.velR, S, d);
     else if (!dump->out)
         hipLaunchKernelGGL(k_density<1>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.posR, a.velR, n, P, d);
+                           a.posR, a.velR, S, d);
     else
         hipLaunchKernelGGL(k_density<2>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.posR, a.velR, n, P, d);
+                           a.posR, a.velR, S, d);
 }
 
 // ----------------------------------------------------- force + integrate
@@ -539,7 +565,7 @@ void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c
 //   pressure = -dir m (p_i + p_j) / (2 rho_j) spikyGrad (h - r)^2
 //   visco    = mu m (v_j - v_i) / rho_j spikyLap (h - r)
 // with dir = (x_j - x_i) / r.  Both are evaluated as one scalar coefficient
-// times a vector, from 1 / rho_j precomputed by the density pass.
+// times a vector, from 0.5 / rho_j precomputed by the density pass.
 __device__ inline void integrate_and_walls(float4 &x, float4 &v, float fx, float fy, float fz,
                                            float rho, const StepParams &P)
 {
@@ -593,15 +619,14 @@ __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
     const uint32_t *__restrict__ cellcB, const uint32_t *__restrict__ cstart,
     const float *__restrict__ rhoB, float4 *__restrict__ posA, float4 *__restrict__ velA,
     uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA, uint32_t *__restrict__ counts,
-    float4 *__restrict__ force_out, uint32_t n, StepParams P)
+    float4 *__restrict__ force_out, SceneSet S)
 {
-    const uint32_t first = xcd_block() * kStencilBlock;
-    if (first >= n)
-        return;  // whole block
-    const uint32_t s = first + threadIdx.x;
-    if (s >= n)
+    Lane L;
+    if (!lane_of<kStencilBlock>(S, L))
         return;
-    float4 pi = posR[s];  // w = 0.5 / rho_i
+    const StepParams &P = *L.P;
+    const uint32_t s = L.slot;
+    float4 pi = posR[s];        // w = 0.5 / rho_i
     const float4 vr = velR[s];  // w = p_i
     float4 vi = make_float4(vr.x, vr.y, vr.z, velB[s].w);  // w = particle id
     const uint32_t cc = cellcB[s];
@@ -609,65 +634,51 @@ __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
     const RefCell ci = ref_cell(pi, P);
     const float rho_i = rhoB[s];
     const float p_i = vr.w;
+    const float h = P.h, h2 = P.h2, neg_m_spg = P.neg_m_spg;
     const float mu2_m_lap = 2.0f * P.mu_m_lap;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 
-    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc);
-    const int x0 = gx > 0 ? gx - 1 : 0;
-    const int x1 = gx + 1 < P.nx ? gx + 1 : P.nx - 1;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = gz + dz;
-        if (z < 0 || z >= P.nz)
-            continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = gy + dy;
-            if (y < 0 || y >= P.ny)
-                continue;
-            const uint32_t row = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
-            const uint32_t b = cstart[row + (uint32_t)x0];
-            const uint32_t e = cstart[row + (uint32_t)x1 + 1u];
-            for (uint32_t jj = b; jj < e; jj += kForceChunk) {
-                uint32_t mask = 0u;
+    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+        for (uint32_t jj = b; jj < e; jj += kForceChunk) {
+            uint32_t mask = 0u;
 #pragma unroll
-                for (int k = 0; k < kForceChunk; ++k) {
-                    const uint32_t j = jj + (uint32_t)k;
-                    const float4 q = posR[j < e ? j : e - 1u];
-                    float dx, dy_, dz_;
-                    const float d2 = dist2_ref(pi, q, dx, dy_, dz_);
-                    const bool pass = (j < e) && (j != s) && (d2 < P.h2);
-                    mask |= pass ? (1u << k) : 0u;
-                }
-                while (mask != 0u) {
-                    const uint32_t k = (uint32_t)__builtin_ctz(mask);
-                    mask &= mask - 1u;
-                    const uint32_t j = jj + k;
-                    const float4 q = posR[j];  // w = 0.5 / rho_j
-                    const float4 w = velR[j];  // w = p_j
-                    float dx, dy_, dz_;
-                    const float d2 = dist2_ref(pi, q, dx, dy_, dz_);
-                    const int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z,
-                                                             __float_as_uint(posB[j].w) & 0xFFFFu)
-                                       : 1;
-                    // r = sqrt(d2) correctly rounded and hr = h - r exactly as
-                    // the reference forms them: near the support edge h - r
-                    // cancels and any error in r would be amplified.
-                    const float r = sqrt_rn(d2);
-                    const float rinv = __builtin_amdgcn_rcpf(r);
-                    const float hr = P.h - r;
-                    float cp = P.neg_m_spg * (p_i + w.w) * q.w * (hr * hr) * rinv;
-                    float cv = mu2_m_lap * hr * q.w;
-                    cp *= (float)m;
-                    cv *= (float)m;
-                    fx = fmaf(cp, dx, fx);
-                    fy = fmaf(cp, dy_, fy);
-                    fz = fmaf(cp, dz_, fz);
-                    fx = fmaf(cv, w.x - vi.x, fx);
-                    fy = fmaf(cv, w.y - vi.y, fy);
-                    fz = fmaf(cv, w.z - vi.z, fz);
-                }
+            for (int k = 0; k < kForceChunk; ++k) {
+                const uint32_t j = jj + (uint32_t)k;
+                const float4 q = posR[j < e ? j : e - 1u];
+                float dx, dy, dz;
+                const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                const bool pass = (j < e) && (j != s) && (d2 < h2);
+                mask |= pass ? (1u << k) : 0u;
+            }
+            while (mask != 0u) {
+                const uint32_t k = (uint32_t)__builtin_ctz(mask);
+                mask &= mask - 1u;
+                const uint32_t j = jj + k;
+                const float4 q = posR[j];  // w = 0.5 / rho_j
+                const float4 w = velR[j];  // w = p_j
+                float dx, dy, dz;
+                const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                const float m = slow ? (float)bucket_multiplicity(ci.x, ci.y, ci.z,
+                                                                  __float_as_uint(posB[j].w) & 0xFFFFu)
+                                     : 1.0f;
+                // r = sqrt(d2) correctly rounded and hr = h - r exactly as
+                // the reference forms them: near the support edge h - r
+                // cancels and any error in r would be amplified.
+                const float r = sqrt_rn(d2);
+                const float rinv = __builtin_amdgcn_rcpf(r);
+                const float hr = h - r;
+                const float am = q.w * m;
+                const float cp = neg_m_spg * (p_i + w.w) * am * (hr * hr) * rinv;
+                const float cv = mu2_m_lap * hr * am;
+                fx = fmaf(cp, dx, fx);
+                fy = fmaf(cp, dy, fy);
+                fz = fmaf(cp, dz, fz);
+                fx = fmaf(cv, w.x - vi.x, fx);
+                fy = fmaf(cv, w.y - vi.y, fy);
+                fz = fmaf(cv, w.z - vi.z, fz);
             }
         }
-    }
+    });
     if (WRITE_FORCE)
         force_out[s] = make_float4(fx, fy, fz, 0.0f);
     integrate_and_walls(pi, vi, fx, fy, fz, rho_i, P);
@@ -684,13 +695,13 @@ __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
 }
 
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                            uint32_t n, const StepParams &P, float4 *force_out, bool next_keys)
+                            const SceneSet &S, float4 *force_out, bool next_keys)
 {
-    dim3 g(blocks_for(n, kStencilBlock)), b(kStencilBlock);
+    dim3 g(blocks_for_waves(S.nwaves, kStencilBlock)), b(kStencilBlock);
 #define SPH_LAUNCH_FI(WF, NK)                                                                   \
     hipLaunchKernelGGL((k_force_integrate<WF, NK>), g, b, 0, st, a.posB, a.velB, a.posR, a.velR, \
                        a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,  \
-                       force_out, n, P)
+                       force_out, S)
     if (force_out && next_keys)
         SPH_LAUNCH_FI(true, true);
     else if (force_out)
@@ -763,13 +774,9 @@ void launch_h16_scatter(hipStream_t st, const ParticleArrays &a, const CellArray
                        c16.cstart, perm16, n);
 }
 
-struct AosOut {
-    float v[15];
-};
-
 // Output slot o: the particle perm16[o] lands at its bucket start plus the
 // number of bucket members with a smaller id (ties of std::sort resolved by
-// input order).
+// input order).  Buckets above kBigCellSortMax keep arrival order.
 __global__ void k_pack_out(const uint32_t *__restrict__ perm16, const uint32_t *__restrict__ cstart16,
                            const float4 *__restrict__ posB, const float4 *__restrict__ velB,
                            const float *__restrict__ rhoB, const float4 *__restrict__ velR,
@@ -786,7 +793,7 @@ __global__ void k_pack_out(const uint32_t *__restrict__ perm16, const uint32_t *
     const uint32_t id = __float_as_uint(velB[s].w);
     const uint32_t b = cstart16[h], e = cstart16[h + 1];
     uint32_t d;
-    if (e - b <= kStableCellMax) {
+    if (e - b <= kBigCellSortMax) {
         uint32_t r = 0;
         for (uint32_t t = b; t < e; ++t)
             r += (__float_as_uint(velB[perm16[t]].w) < id) ? 1u : 0u;
@@ -818,27 +825,29 @@ __global__ void k_pack_out(const uint32_t *__restrict__ perm16, const uint32_t *
 
 void launch_pack_out(hipStream_t st, const ParticleArrays &a, const CellArrays &c16,
                      const uint32_t *perm16, const float4 *force, const void *aos_in, uint32_t n,
-                     const StepParams &P, void *aos_out, float *xf_out)
+                     float xf_scale, void *aos_out, float *xf_out)
 {
     hipLaunchKernelGGL(k_pack_out, dim3(blocks_for(n)), dim3(kBlock), 0, st, perm16, c16.cstart,
                        a.posB, a.velB, a.rhoB, a.velR, force, a.posA, a.velA,
-                       (const float *)aos_in, (float *)aos_out, xf_out, n, P.xf_scale);
+                       (const float *)aos_in, (float *)aos_out, xf_out, n, xf_scale);
 }
 
-// Resident download: slot s of the A arrays (device order).  Bucket, density
-// and pressure belong to the last step (B slot s is the same particle).
+// Resident download: slots [first, first + n) of the A arrays (device
+// order).  Bucket, density and pressure belong to the last step (B slot s is
+// the same particle).
 __global__ void k_pack_state(const float4 *__restrict__ posA, const float4 *__restrict__ velA,
                              const float4 *__restrict__ posB, const float *__restrict__ rhoB,
                              const float4 *__restrict__ velR, const float4 *__restrict__ force,
-                             float *__restrict__ aos_out, float *__restrict__ xf_out, uint32_t n,
-                             float sc)
+                             float *__restrict__ aos_out, float *__restrict__ xf_out, uint32_t first,
+                             uint32_t n, float sc)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n)
+    uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n)
         return;
+    const uint32_t s = first + o;
     const float4 x = posA[s], v = velA[s];
     const float4 f = force ? force[s] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float *dst = aos_out + (size_t)s * 15;
+    float *dst = aos_out + (size_t)o * 15;
     dst[0] = x.x;
     dst[1] = x.y;
     dst[2] = x.z;
@@ -855,14 +864,14 @@ __global__ void k_pack_state(const float4 *__restrict__ posA, const float4 *__re
     dst[13] = velR[s].w;
     dst[14] = __uint_as_float(__float_as_uint(posB[s].w) & 0xFFFFu);
     if (xf_out)
-        write_transform(xf_out + (size_t)s * 16, x.x, x.y, x.z, sc);
+        write_transform(xf_out + (size_t)o * 16, x.x, x.y, x.z, sc);
 }
 
-void launch_pack_state(hipStream_t st, const ParticleArrays &a, const float4 *force, uint32_t n,
-                       const StepParams &P, void *aos_out, float *xf_out)
+void launch_pack_state(hipStream_t st, const ParticleArrays &a, const float4 *force, uint32_t first,
+                       uint32_t n, float xf_scale, void *aos_out, float *xf_out)
 {
     hipLaunchKernelGGL(k_pack_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, a.posA, a.velA,
-                       a.posB, a.rhoB, a.velR, force, (float *)aos_out, xf_out, n, P.xf_scale);
+                       a.posB, a.rhoB, a.velR, force, (float *)aos_out, xf_out, first, n, xf_scale);
 }
 
 // ------------------------------------------------------------------ copy

@@ -40,14 +40,20 @@ struct DevBuf {
 // All device arrays of one particle set (see sph_kernels.h for the layout).
 struct Workspace {
     DevBuf posA, velA, posB, velB, cellcA, rankA, cellcB, perm, rhoB, posR, velR;
-    DevBuf counts, cstart, partials;
+    DevBuf counts, cstart, partials, big;
     DevBuf force;
+    DevBuf params, tasks;  // scene table of the batch living in this workspace
     uint32_t ncells_padded = 0;
-    bool counts_dirty = true;
+    uint32_t big_cap = 0;
+    uint32_t nwaves = 0;
     int reserve_particles(uint32_t n);
     int reserve_cells(uint64_t ncells);
+    // Uploads the scene table (p_begin/p_end/cell_base filled in) and builds
+    // one 64-slot wave task per started 64 slots of every scene.
+    int set_scenes(const std::vector<StepParams> &scenes, hipStream_t st);
     ParticleArrays particles() const;
     CellArrays cells() const;
+    SceneSet scene_set() const;
     void release_all();
 };
 
@@ -73,7 +79,8 @@ struct StepTimer {
     ~StepTimer() { clear(); }
 };
 
-void enqueue_step(hipStream_t st, Workspace &w, uint32_t n, const StepParams &P, float4 *force_out,
-                  bool next_keys, StepTimer *timer);
+// One step on the workspace (keys of posA valid, counts holding the
+// per-cell totals).  `timer` (optional) records an event after each kernel.
+void enqueue_step(hipStream_t st, Workspace &w, float4 *force_out, bool next_keys, StepTimer *timer);
 
 }  // namespace sph

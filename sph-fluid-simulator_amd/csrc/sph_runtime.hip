@@ -222,6 +222,11 @@ void DevBuf::release()
 int Workspace::reserve_particles(uint32_t n)
 {
     int rc;
+    // cells holding more than kSmallCell particles: at most n / (kSmallCell + 1)
+    const uint32_t cap = n / (kSmallCell + 1u) + 1u;
+    if ((rc = big.reserve(4ull * (cap + 1u))))
+        return rc;
+    big_cap = (uint32_t)(big.bytes / 4 - 1);
     if ((rc = posA.reserve(sizeof(float4) * (size_t)n)) || (rc = velA.reserve(sizeof(float4) * (size_t)n)) ||
         (rc = posB.reserve(sizeof(float4) * (size_t)n)) || (rc = velB.reserve(sizeof(float4) * (size_t)n)) ||
         (rc = cellcA.reserve(4ull * n)) || (rc = rankA.reserve(4ull * n)) ||
@@ -240,14 +245,34 @@ int Workspace::reserve_cells(uint64_t ncells)
     if (padded > (1ull << 31))
         return fail(SPH_E_LIMIT, "grid of %llu cells exceeds the 2^31 limit", (unsigned long long)ncells);
     int rc;
-    bool grew = padded * 4 > counts.bytes;
     if ((rc = counts.reserve(4 * padded)) || (rc = cstart.reserve(4 * (padded + 1))) ||
         (rc = partials.reserve(4 * (padded / kScanChunk))))
         return rc;
-    if (grew || padded != ncells_padded) {
-        ncells_padded = (uint32_t)padded;
-        counts_dirty = true;
-    }
+    ncells_padded = (uint32_t)padded;
+    return SPH_OK;
+}
+
+int Workspace::set_scenes(const std::vector<StepParams> &scenes, hipStream_t st)
+{
+    std::vector<WaveTask> t;
+    for (uint32_t k = 0; k < scenes.size(); ++k)
+        for (uint32_t f = scenes[k].p_begin; f < scenes[k].p_end; f += 64u)
+            t.push_back(WaveTask{k, f});
+    int rc;
+    if ((rc = params.reserve(sizeof(StepParams) * std::max<size_t>(scenes.size(), 1))) ||
+        (rc = tasks.reserve(sizeof(WaveTask) * std::max<size_t>(t.size(), 1))))
+        return rc;
+    hipError_t e = hipSuccess;
+    if (!scenes.empty())
+        e = hipMemcpyAsync(params.ptr, scenes.data(), sizeof(StepParams) * scenes.size(),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !t.empty())
+        e = hipMemcpyAsync(tasks.ptr, t.data(), sizeof(WaveTask) * t.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);  // the host vectors go out of scope
+    if (e != hipSuccess)
+        return hip_fail(e, "scene table upload");
+    nwaves = (uint32_t)t.size();
     return SPH_OK;
 }
 
@@ -274,40 +299,52 @@ CellArrays Workspace::cells() const
     c.counts = (uint32_t *)counts.ptr;
     c.cstart = (uint32_t *)cstart.ptr;
     c.partials = (uint32_t *)partials.ptr;
+    c.big = (uint32_t *)big.ptr;
+    c.big_cap = big_cap;
     c.ncells_padded = ncells_padded;
     return c;
+}
+
+SceneSet Workspace::scene_set() const
+{
+    SceneSet S;
+    S.params = (const StepParams *)params.ptr;
+    S.tasks = (const WaveTask *)tasks.ptr;
+    S.nwaves = nwaves;
+    return S;
 }
 
 void Workspace::release_all()
 {
     for (DevBuf *b : {&posA, &velA, &posB, &velB, &cellcA, &rankA, &cellcB, &perm, &rhoB, &posR, &velR,
-                      &counts, &cstart, &partials, &force})
+                      &counts, &cstart, &partials, &big, &force, &params, &tasks})
         b->release();
     ncells_padded = 0;
+    big_cap = 0;
+    nwaves = 0;
 }
 
-// One step on the workspace, keys of posA already valid (counts hold the
-// per-cell totals).  `timer` (optional) records an event after each kernel.
-void enqueue_step(hipStream_t st, Workspace &w, uint32_t n, const StepParams &P, float4 *force_out,
-                  bool next_keys, StepTimer *timer)
+void enqueue_step(hipStream_t st, Workspace &w, float4 *force_out, bool next_keys, StepTimer *timer)
 {
     ParticleArrays a = w.particles();
     CellArrays c = w.cells();
+    SceneSet S = w.scene_set();
     if (timer)
         timer->mark(st, nullptr);
-    launch_scan(st, c, n);
+    launch_scan(st, c);
     if (timer)
         timer->mark(st, "scan");
-    launch_scatter(st, a, c, n, P);
+    launch_scatter(st, a, c, S);
+    launch_sort_big_cells(st, a, c);
     if (timer)
         timer->mark(st, "scatter");
-    launch_reorder(st, a, c, n, P);
+    launch_reorder(st, a, c, S);
     if (timer)
         timer->mark(st, "reorder");
-    launch_density(st, a, c, n, P, nullptr);
+    launch_density(st, a, c, S, nullptr);
     if (timer)
         timer->mark(st, "density");
-    launch_force_integrate(st, a, c, n, P, force_out, next_keys);
+    launch_force_integrate(st, a, c, S, force_out, next_keys);
     if (timer)
         timer->mark(st, "force_integrate");
 }
@@ -387,15 +424,19 @@ int stateless_prepare(Stateless &S, const sph_particle_v1 *particles, uint32_t n
     P = make_params(*settings, box, dt);
     uint64_t max_cells = std::max<uint64_t>(8ull * n, 1ull << 18);
     make_grid(P, lo, hi, 1.0, max_cells);
-    if ((rc = w.reserve_cells((uint64_t)P.nx * P.ny * P.nz)))
+    P.cell_base = 0;
+    P.p_begin = 0;
+    P.p_end = n;
+    if ((rc = w.reserve_cells((uint64_t)P.nx * P.ny * P.nz)) || (rc = w.set_scenes({P}, st)))
         return rc;
     CellArrays c = w.cells();
+    SceneSet set = w.scene_set();
     launch_fill_u32(st, c.counts, 0u, c.ncells_padded);
-    w.counts_dirty = false;
-    launch_init_keys(st, a, c, n, P);
-    launch_scan(st, c, n);
-    launch_scatter(st, a, c, n, P);
-    launch_reorder(st, a, c, n, P);
+    launch_init_keys(st, a, c, set);
+    launch_scan(st, c);
+    launch_scatter(st, a, c, set);
+    launch_sort_big_cells(st, a, c);
+    launch_reorder(st, a, c, set);
     return SPH_OK;
 }
 
@@ -414,6 +455,8 @@ CellArrays cells16(Stateless &S)
     c.counts = (uint32_t *)S.counts16.ptr;
     c.cstart = (uint32_t *)S.cstart16.ptr;
     c.partials = (uint32_t *)S.partials16.ptr;
+    c.big = nullptr;
+    c.big_cap = 0;
     c.ncells_padded = 65536;
     return c;
 }
@@ -454,15 +497,16 @@ int update_particles(sph_particle_v1 *particles, float *transforms, size_t n_,
     hipStream_t st = S.stream;
     ParticleArrays a = w.particles();
     CellArrays c = w.cells();
-    launch_density(st, a, c, n, P, nullptr);
-    launch_force_integrate(st, a, c, n, P, (float4 *)w.force.ptr, /*next_keys=*/false);
+    SceneSet set = w.scene_set();
+    launch_density(st, a, c, set, nullptr);
+    launch_force_integrate(st, a, c, set, (float4 *)w.force.ptr, /*next_keys=*/false);
     CellArrays c16 = cells16(S);
     launch_fill_u32(st, c16.counts, 0u, 65536);
     launch_h16_rank(st, a, c16, n);
-    launch_scan(st, c16, n);
+    launch_scan(st, c16);
     launch_h16_scatter(st, a, c16, n, (uint32_t *)S.perm16.ptr);
     launch_pack_out(st, a, c16, (const uint32_t *)S.perm16.ptr, (const float4 *)w.force.ptr,
-                    S.aos_in.ptr, n, P, S.aos_out.ptr, transforms ? (float *)S.xf.ptr : nullptr);
+                    S.aos_in.ptr, n, P.xf_scale, S.aos_out.ptr, transforms ? (float *)S.xf.ptr : nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return hip_fail(e, "kernel launch");
@@ -512,8 +556,9 @@ int neighbors(const sph_particle_v1 *particles, size_t n_, const sph_settings_v1
     DevBuf dcount, doff, dout;
     if ((rc = dcount.reserve(4ull * n)) || (rc = doff.reserve(8ull * (n + 1))))
         return rc;
+    SceneSet set = w.scene_set();
     NeighborDump d{(uint32_t *)dcount.ptr, nullptr, nullptr};
-    launch_density(st, a, c, n, P, &d);
+    launch_density(st, a, c, set, &d);
     std::vector<uint32_t> cnt(n), ids(n);
     std::vector<float4> velB(n);
     hipError_t e = hipMemcpyAsync(cnt.data(), dcount.ptr, 4ull * n, hipMemcpyDeviceToHost, st);
@@ -555,7 +600,7 @@ int neighbors(const sph_particle_v1 *particles, size_t n_, const sph_settings_v1
     if (e != hipSuccess)
         return hip_fail(e, "neighbour offsets");
     NeighborDump d2{(uint32_t *)dcount.ptr, (const uint64_t *)doff.ptr, (uint32_t *)dout.ptr};
-    launch_density(st, a, c, n, P, &d2);
+    launch_density(st, a, c, set, &d2);
     std::vector<uint32_t> rows(tot);
     e = hipMemcpyAsync(rows.data(), dout.ptr, 4ull * tot, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess)

@@ -112,6 +112,16 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_hip_profile": (
             c_int, [c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_int), c_char_p, c_size_t]),
         "sph_hip_copy_bench": (c_int, [c_size_t, c_int, POINTER(c_double)]),
+        "sph_batch_create": (c_int, [POINTER(SceneDesc), c_uint32, POINTER(c_void_p)]),
+        "sph_batch_destroy": (c_int, [c_void_p]),
+        "sph_batch_upload": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
+        "sph_batch_step": (c_int, [c_void_p, c_int]),
+        "sph_batch_sync": (c_int, [c_void_p]),
+        "sph_batch_download": (c_int, [c_void_p, c_uint32] + [c_void_p] * 5),
+        "sph_batch_set_graph": (c_int, [c_void_p, c_int]),
+        "sph_batch_scene_count": (c_uint32, [c_void_p]),
+        "sph_batch_profile": (
+            c_int, [c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_int), c_char_p, c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -282,6 +292,81 @@ class Context:
         nk = c_int(0)
         names = ctypes.create_string_buffer(1024)
         _check(self._lib.sph_hip_profile(self._h, nsteps, ms, 32, ctypes.byref(nk), names, 1024))
+        labels = names.raw.split(b"\0")[: nk.value]
+        return {labels[k].decode(): float(ms[k]) for k in range(nk.value)}
+
+
+def scene_desc(n: int, s: Settings, box_: Box, dt: float, cell_scale: float = 1.0,
+               keep_force: bool = False) -> SceneDesc:
+    d = SceneDesc()
+    d.n = n
+    d.dt = dt
+    d.settings = s
+    d.box = box_
+    d.cell_scale = cell_scale
+    d.flags = SPH_SCENE_KEEP_FORCE if keep_force else 0
+    return d
+
+
+class Batch:
+    """Independent scenes advanced together, one launch per pass."""
+
+    def __init__(self, descs):
+        self._lib = load()
+        arr = (SceneDesc * len(descs))(*descs)
+        h = c_void_p()
+        _check(self._lib.sph_batch_create(arr, len(descs), ctypes.byref(h)))
+        self._h = h
+        self.counts = [d.n for d in descs]
+
+    def close(self):
+        if self._h:
+            self._lib.sph_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, scene: int, pos: np.ndarray, vel: np.ndarray | None = None):
+        n = self.counts[scene]
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(n, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        _check(self._lib.sph_batch_upload(self._h, scene, _ptr(pos), _ptr(vel)))
+
+    def step(self, nsteps: int = 1):
+        _check(self._lib.sph_batch_step(self._h, nsteps))
+
+    def sync(self):
+        _check(self._lib.sph_batch_sync(self._h))
+
+    def set_graph(self, enable: bool):
+        _check(self._lib.sph_batch_set_graph(self._h, 1 if enable else 0))
+
+    def download(self, scene: int):
+        n = self.counts[scene]
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        rho = np.zeros(n, np.float32)
+        prs = np.zeros(n, np.float32)
+        ids = np.zeros(n, np.uint32)
+        _check(self._lib.sph_batch_download(self._h, scene, _ptr(pos), _ptr(vel), _ptr(rho), _ptr(prs),
+                                            _ptr(ids)))
+        return {"pos": pos, "vel": vel, "rho": rho, "pressure": prs, "ids": ids}
+
+    def profile(self, nsteps: int = 10) -> dict:
+        ms = (c_double * 32)()
+        nk = c_int(0)
+        names = ctypes.create_string_buffer(1024)
+        _check(self._lib.sph_batch_profile(self._h, nsteps, ms, 32, ctypes.byref(nk), names, 1024))
         labels = names.raw.split(b"\0")[: nk.value]
         return {labels[k].decode(): float(ms[k]) for k in range(nk.value)}
 

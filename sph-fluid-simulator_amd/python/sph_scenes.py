@@ -126,3 +126,29 @@ def reference_app_scene(cube_width: int = 15) -> Scene:
     pos = reference_init(cube_width, h)
     return Scene(f"reference-app-{cube_width}^3", pos, np.zeros_like(pos), 0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2,
                  0.003, (-8.0, 0.0, -8.0), (8.0, math.inf, 8.0), steps=1000)
+
+
+def batched_workload(nscenes: int = 256, seed: int = 1024, dt: float = 0.002):
+    """The batched workload: `nscenes` independent scenes, particle counts
+    log-uniform in [1k, 64k], each a jittered lattice block (spacing
+    0.8 / ceil(cbrt(N)), h = 2.1 spacing, i.e. ~32 interior neighbours) at a
+    random corner of its own [-1,1]^3 box, viscosity ~ U[0.5, 2], gas
+    constant ~ U[0.5, 3], mass 0.02 (h / 0.15)^3, 500 steps."""
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for k in range(nscenes):
+        n = int(round(math.exp(rng.uniform(math.log(1000.0), math.log(65536.0)))))
+        side = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+        sp = 0.8 / side
+        h = 2.1 * sp
+        corner = rng.integers(0, 2, size=3)
+        lo = -1.0 + h + 0.5 * sp
+        hi = 1.0 - h - 0.5 * sp - (side - 1) * sp
+        origin = [lo if c == 0 else hi for c in corner]
+        pos = lattice((side, side, side), sp, origin, 0.01 * h, seed + 7919 * (k + 1))[:n]
+        mu = float(rng.uniform(0.5, 2.0))
+        gas = float(rng.uniform(0.5, 3.0))
+        mass = float(F32(0.02 * (h / 0.15) ** 3))
+        scenes.append(Scene(f"batched-{k}-{n}", pos, np.zeros_like(pos), mass, 1000.0, gas, mu, h, -9.8, 0.2,
+                            dt, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), steps=500))
+    return scenes
