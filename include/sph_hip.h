@@ -86,6 +86,11 @@ typedef struct sph_scene_desc_v1 {
  * per step of extra traffic). */
 #define SPH_SCENE_KEEP_FORCE 1u
 
+/* Every wave of the density and force passes walks its candidates per lane
+ * instead of choosing the wave-uniform walk where it pays.  The results are
+ * bit-identical either way; the flag exists for tests and tuning. */
+#define SPH_SCENE_LANE_WALK 2u
+
 /* ---- host-only helpers (no device work) --------------------------------- */
 
 int sph_abi_version(void);

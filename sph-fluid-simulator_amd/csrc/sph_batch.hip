@@ -22,7 +22,7 @@
 namespace sph {
 
 namespace {
-constexpr uint32_t kKnownFlags = SPH_SCENE_KEEP_FORCE;
+constexpr uint32_t kKnownFlags = SPH_SCENE_KEEP_FORCE | SPH_SCENE_LANE_WALK;
 }
 
 int validate_desc(const sph_scene_desc_v1 &d)
@@ -112,6 +112,7 @@ int Batch::init(const sph_scene_desc_v1 *descs, uint32_t k)
         if (total > kMaxParticles)
             return fail(SPH_E_LIMIT, "batch: %llu particles exceed the 2^30 limit", (unsigned long long)total);
         b.P.p_end = (uint32_t)total;
+        b.P.lane_walk = (descs[s].flags & SPH_SCENE_LANE_WALK) ? 1u : 0u;
         keep_force = keep_force || (descs[s].flags & SPH_SCENE_KEEP_FORCE);
         scenes.push_back(b);
     }

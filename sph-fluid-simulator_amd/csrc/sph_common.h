@@ -72,6 +72,8 @@ struct StepParams {
     uint32_t cell_base;  // first linear cell of this scene
     // slots [p_begin, p_end) of the particle arrays belong to this scene
     uint32_t p_begin, p_end;
+    // 1: every wave walks its candidates per lane (never the union walk)
+    uint32_t lane_walk;
 };
 
 SPH_HD inline uint32_t hash16(uint32_t cx, uint32_t cy, uint32_t cz)

@@ -38,7 +38,7 @@ constexpr uint32_t kScanChunk = kBlock * kScanItems;  // cells per scan block
 constexpr uint32_t kSmallCell = 64;        // cells up to this size: stable rank by counting
 constexpr uint32_t kBigCellSortMax = 16384;  // larger cells keep arrival order
 constexpr int kBigSortBlock = 1024;        // threads of the big-cell sort
-constexpr int kBigSortBlocks = 64;         // blocks walking the big-cell list
+constexpr int kBigSortBlocks = 256;        // blocks walking the big-cell list
 
 struct ParticleArrays {
     float4 *posA, *velA, *posB, *velB;

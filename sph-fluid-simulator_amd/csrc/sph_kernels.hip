@@ -23,6 +23,8 @@
 // contract (tolerance-checked sums).
 #include "sph_kernels.h"
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace sph {
@@ -44,6 +46,7 @@ __device__ inline uint32_t xcd_block()
 struct Lane {
     const StepParams *P;
     uint32_t slot;
+    uint32_t first;  // first slot of the wave (wave-uniform)
 };
 
 template <int BLOCK>
@@ -56,8 +59,30 @@ __device__ inline bool lane_of(const SceneSet &S, Lane &out)
     const uint32_t scene = __builtin_amdgcn_readfirstlane(t.scene);
     const uint32_t first = __builtin_amdgcn_readfirstlane(t.first);
     out.P = S.params + scene;
+    out.first = first;
     out.slot = first + (threadIdx.x & 63u);
     return out.slot < out.P->p_end;
+}
+
+// As lane_of, but every lane of a live wave stays live: lanes past the
+// scene's last slot mirror the wave's first slot (always a valid slot) and
+// report active = false, so wave-wide votes and shuffles see defined data.
+// Returns false only for waves past the task list (the whole wave).
+template <int BLOCK>
+__device__ inline bool wave_lanes(const SceneSet &S, Lane &out, bool &active)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
+    if (w >= S.nwaves)
+        return false;
+    const WaveTask t = S.tasks[w];
+    const uint32_t scene = __builtin_amdgcn_readfirstlane(t.scene);
+    const uint32_t first = __builtin_amdgcn_readfirstlane(t.first);
+    out.P = S.params + scene;
+    out.first = first;
+    const uint32_t slot = first + (threadIdx.x & 63u);
+    active = slot < out.P->p_end;
+    out.slot = active ? slot : first;
+    return true;
 }
 
 inline uint32_t blocks_for_waves(uint32_t nwaves, int block) { return (nwaves + block / 64 - 1) / (block / 64); }
@@ -480,6 +505,121 @@ __device__ inline void for_each_row(const uint32_t *__restrict__ cstart, uint32_
     }
 }
 
+__device__ inline int wave_min_i(int x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        x = min(x, __shfl_xor(x, d, 64));
+    return __builtin_amdgcn_readfirstlane(x);
+}
+
+__device__ inline int wave_max_i(int x)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+        x = max(x, __shfl_xor(x, d, 64));
+    return __builtin_amdgcn_readfirstlane(x);
+}
+
+// A box of grid cells [x0, x1] x [y0, y1] x [z0, z1] (inside the grid)
+// whose rows a wave walks with one candidate index for all of its lanes.
+struct UnionRows {
+    int x0, x1, y0, y1, z0, z1;
+};
+
+__device__ inline uint32_t row_base(const StepParams &P, int y, int z)
+{
+    return P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
+}
+
+// Chooses between the two ways a wave walks its candidates:
+//  * per lane: each lane walks its own 3x3x3 stencil (gathers);
+//  * union: all lanes walk the union of their stencils together, so the
+//    candidate index is wave-uniform and its record is a scalar load
+//    broadcast to every lane.
+// Union is taken when all lanes sit in one cell (the dense regime: one
+// cell holds many waves), or when their cells span at most 4 x 2 x 2 and
+// the union holds at most 5/4 of the largest own stencil.  A candidate the
+// union adds to a lane's own stencil lies in a cell two or more cells away,
+// more than H >= h (1 + 2^-10) apart along some axis, so its float squared
+// distance is above h^2 and it never passes: both walks visit a lane's
+// passing candidates in the same order (rows z-major, then y, slots
+// ascending) and give bit-identical sums.
+__device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t cc, const StepParams &P,
+                                  UnionRows &U)
+{
+    if (P.lane_walk)
+        return false;
+    const uint32_t cell = cc & ~kSlowBit;
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(cell);
+    int x0, x1, y0, y1, z0, z1;
+    const bool one = __all(cell == c0);
+    if (one) {
+        x0 = x1 = cell_x(c0);
+        y0 = y1 = cell_y(c0);
+        z0 = z1 = cell_z(c0);
+    } else {
+        const int gx = cell_x(cell), gy = cell_y(cell), gz = cell_z(cell);
+        x0 = wave_min_i(gx);
+        x1 = wave_max_i(gx);
+        y0 = wave_min_i(gy);
+        y1 = wave_max_i(gy);
+        z0 = wave_min_i(gz);
+        z1 = wave_max_i(gz);
+        if (x1 - x0 > 3 || y1 - y0 > 1 || z1 - z0 > 1)
+            return false;
+    }
+    U.x0 = x0 > 0 ? x0 - 1 : 0;
+    U.x1 = x1 + 1 < P.nx ? x1 + 1 : P.nx - 1;
+    U.y0 = y0 > 0 ? y0 - 1 : 0;
+    U.y1 = y1 + 1 < P.ny ? y1 + 1 : P.ny - 1;
+    U.z0 = z0 > 0 ? z0 - 1 : 0;
+    U.z1 = z1 + 1 < P.nz ? z1 + 1 : P.nz - 1;
+    if (one)
+        return true;
+    uint32_t un = 0;
+    for (int z = U.z0; z <= U.z1; ++z)
+        for (int y = U.y0; y <= U.y1; ++y) {
+            const uint32_t r0 = row_base(P, y, z);
+            un += cstart[r0 + (uint32_t)U.x1 + 1u] - cstart[r0 + (uint32_t)U.x0];
+        }
+    uint32_t own = 0;
+    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+    const uint32_t own_max = (uint32_t)wave_max_i((int)own);
+    return 4ull * un <= 5ull * own_max;
+}
+
+// Walks the wave-uniform range [b, e) as three runs: candidates before the
+// wave's own slots [lo, lo + 64), the own slots (where a lane meets
+// itself and must skip it), and those after.  run(j0, j1, self) walks
+// [j0, j1); self says whether the run can hold a lane's own slot.
+template <class Run>
+__device__ inline void split_self(uint32_t b, uint32_t e, uint32_t lo, Run run)
+{
+    const uint32_t hi = lo + 64u;
+    const uint32_t m0 = min(max(b, lo), e), m1 = min(max(b, hi), e);
+    if (b < m0)
+        run(b, m0, std::false_type{});
+    if (m0 < m1)
+        run(m0, m1, std::true_type{});
+    if (m1 < e)
+        run(m1, e, std::false_type{});
+}
+
+// Calls row(b, e) for each row of the union box, in the order for_each_row
+// visits rows.  b and e are wave-uniform.
+template <class Row>
+__device__ inline void for_each_union_row(const uint32_t *__restrict__ cstart, const UnionRows &U,
+                                          const StepParams &P, Row row)
+{
+    for (int z = U.z0; z <= U.z1; ++z)
+        for (int y = U.y0; y <= U.y1; ++y) {
+            const uint32_t r0 = row_base(P, y, z);
+            row(__builtin_amdgcn_readfirstlane(cstart[r0 + (uint32_t)U.x0]),
+                __builtin_amdgcn_readfirstlane(cstart[r0 + (uint32_t)U.x1 + 1u]));
+        }
+}
+
 // --------------------------------------------------------------- density
 // MODE 0: density + pressure.  MODE 1: also the neighbour count per
 // particle.  MODE 2: also the neighbour ids (multiplicity-repeated).
@@ -492,56 +632,104 @@ __global__ __launch_bounds__(kStencilBlock) void k_density(
     NeighborDump dump)
 {
     Lane L;
-    if (!lane_of<kStencilBlock>(S, L))
+    bool active;
+    if (!wave_lanes<kStencilBlock>(S, L, active))
         return;
-    const StepParams &P = *L.P;
-    const uint32_t s = L.slot;
-    const float4 pi = pos[s];
-    const uint32_t cc = cellc[s];
-    const bool slow = (cc & kSlowBit) != 0u;
-    const RefCell ci = ref_cell(pi, P);
-    const float h2 = P.h2, mp6 = P.mp6;
-    float acc = 0.0f;
-    uint32_t cnt = 0;
-    uint64_t w = (MODE == 2) ? dump.off[s] : 0;
-    auto body = [&](uint32_t j, float4 q) {
-        float dx, dy, dz;
-        float d2 = dist2_ref(pi, q, dx, dy, dz);
-        if (d2 < h2) {
-            int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
-            float t = h2 - d2;
-            float term = mp6 * ((t * t) * t);
-            acc += (float)m * term;
-            if (MODE >= 1)
-                cnt += (uint32_t)m;
-            if (MODE == 2) {
-                uint32_t id = __float_as_uint(vel[j].w);
-                for (int k = 0; k < m; ++k)
-                    dump.out[w++] = id;
+    {
+        const StepParams &P = *L.P;
+        const uint32_t s = L.slot;
+        const float4 pi = pos[s];
+        const uint32_t cc = cellc[s];
+        const bool slow = (cc & kSlowBit) != 0u;
+        const RefCell ci = ref_cell(pi, P);
+        const float h2 = P.h2, mp6 = P.mp6;
+        float acc = 0.0f;
+        uint32_t cnt = 0;
+        uint64_t w = (MODE == 2 && active) ? dump.off[s] : 0;
+        auto body = [&](uint32_t j, float4 q) {
+            float dx, dy, dz;
+            float d2 = dist2_ref(pi, q, dx, dy, dz);
+            if (d2 < h2) {
+                int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
+                float t = h2 - d2;
+                float term = mp6 * ((t * t) * t);
+                acc += (float)m * term;
+                if (MODE >= 1)
+                    cnt += (uint32_t)m;
+                if (MODE == 2 && active) {
+                    uint32_t id = __float_as_uint(vel[j].w);
+                    for (int k = 0; k < m; ++k)
+                        dump.out[w++] = id;
+                }
             }
+        };
+        UnionRows U;
+        if (MODE == 0 && plan_union(cstart, cc, P, U)) {
+            // Wave-uniform candidate j: pos[j] is one scalar load for the wave.
+            const bool any_slow = __any(slow);
+            auto walk = [&](auto slow_tag) {
+                constexpr bool AS = decltype(slow_tag)::value;
+                auto cand = [&](uint32_t j, float4 q, auto self_tag) {
+                    float dx, dy, dz;
+                    const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                    if (d2 < h2 && (!decltype(self_tag)::value || j != s)) {
+                        const float t = h2 - d2;
+                        const float term = mp6 * ((t * t) * t);
+                        if (AS) {
+                            const int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z,
+                                                                     __float_as_uint(q.w) & 0xFFFFu)
+                                               : 1;
+                            acc += (float)m * term;
+                        } else {
+                            acc += term;  // (float)1 * term == term
+                        }
+                    }
+                };
+                auto run = [&](uint32_t j0, uint32_t j1, auto self_tag) {
+                    const float4 *p = pos + j0;
+                    uint32_t j = j0;
+                    for (; j + 4u <= j1; j += 4u, p += 4) {
+                        const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+                        cand(j, q0, self_tag);
+                        cand(j + 1u, q1, self_tag);
+                        cand(j + 2u, q2, self_tag);
+                        cand(j + 3u, q3, self_tag);
+                    }
+                    for (; j < j1; ++j, ++p)
+                        cand(j, p[0], self_tag);
+                };
+                for_each_union_row(cstart, U, P, [&](uint32_t b, uint32_t e) { split_self(b, e, L.first, run); });
+            };
+            if (any_slow)
+                walk(std::integral_constant<bool, true>{});
+            else
+                walk(std::integral_constant<bool, false>{});
+        } else {
+            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+                uint32_t j = b;
+                for (; j + 1u < e; j += 2u) {  // two candidates in flight
+                    const float4 q0 = pos[j];
+                    const float4 q1 = pos[j + 1u];
+                    if (j != s)
+                        body(j, q0);
+                    if (j + 1u != s)
+                        body(j + 1u, q1);
+                }
+                if (j < e && j != s)
+                    body(j, pos[j]);
+            });
         }
-    };
-    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
-        uint32_t j = b;
-        for (; j + 1u < e; j += 2u) {  // two candidates in flight
-            const float4 q0 = pos[j];
-            const float4 q1 = pos[j + 1u];
-            if (j != s)
-                body(j, q0);
-            if (j + 1u != s)
-                body(j + 1u, q1);
-        }
-        if (j < e && j != s)
-            body(j, pos[j]);
-    });
-    const float r = acc + P.self_dens;
-    const float p = P.gas * (r - P.rho0);
-    rho[s] = r;
-    posR[s] = make_float4(pi.x, pi.y, pi.z, 0.5f / r);
-    const float4 vs = vel[s];
-    velR[s] = make_float4(vs.x, vs.y, vs.z, p);
-    if (MODE >= 1)
-        dump.count[s] = cnt;
+        if (!active)
+            return;
+        const float r = acc + P.self_dens;
+        const float p = P.gas * (r - P.rho0);
+        rho[s] = r;
+        posR[s] = make_float4(pi.x, pi.y, pi.z, 0.5f / r);
+        const float4 vs = vel[s];
+        velR[s] = make_float4(vs.x, vs.y, vs.z, p);
+        if (MODE >= 1)
+            dump.count[s] = cnt;
+    }
 }
 
 void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
@@ -608,10 +796,37 @@ __device__ inline void integrate_and_walls(float4 &x, float4 &v, float fx, float
     }
 }
 
-// Candidates are filtered kForceChunk at a time into a per-lane bitmask;
-// the force terms then run only over the set bits, so a wave pays for the
-// most neighbours any of its lanes has in the chunk, not for every
-// candidate some lane accepts.
+// The force of one passing neighbour, accumulated into (fx, fy, fz):
+//   cp = -m spikyGrad (p_i + p_j) m_ij (0.5 / rho_j) (h - r)^2 / r
+//   cv = 2 mu m spikyLap (h - r) m_ij (0.5 / rho_j)
+//   F += cp d + cv (v_j - v_i)
+// q = (x_j, 0.5 / rho_j), w = (v_j, p_j), d = x_j - x_i, am = m_ij (0.5 / rho_j).
+__device__ inline void force_pair(float d2, float dx, float dy, float dz, float am, float4 w, float4 vi,
+                                  float p_i, float h, float neg_m_spg, float mu2_m_lap, float &fx,
+                                  float &fy, float &fz)
+{
+    // r = sqrt(d2) correctly rounded and hr = h - r exactly as the
+    // reference forms them: near the support edge h - r cancels and any
+    // error in r would be amplified.
+    const float r = sqrt_rn(d2);
+    const float rinv = __builtin_amdgcn_rcpf(r);
+    const float hr = h - r;
+    const float cp = neg_m_spg * (p_i + w.w) * am * (hr * hr) * rinv;
+    const float cv = mu2_m_lap * hr * am;
+    fx = fmaf(cp, dx, fx);
+    fy = fmaf(cp, dy, fy);
+    fz = fmaf(cp, dz, fz);
+    fx = fmaf(cv, w.x - vi.x, fx);
+    fy = fmaf(cv, w.y - vi.y, fy);
+    fz = fmaf(cv, w.z - vi.z, fz);
+}
+
+// Two walks, chosen per wave by plan_union, with identical per-lane sums:
+//  * union: one wave-uniform candidate at a time (scalar loads), the pair
+//    term predicated on the lane's own test;
+//  * per lane: candidates filtered kForceChunk at a time into a per-lane
+//    bitmask, the terms then run only over the set bits, so a wave pays for
+//    the most neighbours any of its lanes has in the chunk.
 template <bool WRITE_FORCE, bool NEXT_KEYS>
 __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
     const float4 *__restrict__ posB, const float4 *__restrict__ velB,
@@ -622,76 +837,106 @@ __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
     float4 *__restrict__ force_out, SceneSet S)
 {
     Lane L;
-    if (!lane_of<kStencilBlock>(S, L))
+    bool active;
+    if (!wave_lanes<kStencilBlock>(S, L, active))
         return;
-    const StepParams &P = *L.P;
-    const uint32_t s = L.slot;
-    float4 pi = posR[s];        // w = 0.5 / rho_i
-    const float4 vr = velR[s];  // w = p_i
-    float4 vi = make_float4(vr.x, vr.y, vr.z, velB[s].w);  // w = particle id
-    const uint32_t cc = cellcB[s];
-    const bool slow = (cc & kSlowBit) != 0u;
-    const RefCell ci = ref_cell(pi, P);
-    const float rho_i = rhoB[s];
-    const float p_i = vr.w;
-    const float h = P.h, h2 = P.h2, neg_m_spg = P.neg_m_spg;
-    const float mu2_m_lap = 2.0f * P.mu_m_lap;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    {
+        const StepParams &P = *L.P;
+        const uint32_t s = L.slot;
+        float4 pi = posR[s];        // w = 0.5 / rho_i
+        const float4 vr = velR[s];  // w = p_i
+        float4 vi = make_float4(vr.x, vr.y, vr.z, velB[s].w);  // w = particle id
+        const uint32_t cc = cellcB[s];
+        const bool slow = (cc & kSlowBit) != 0u;
+        const RefCell ci = ref_cell(pi, P);
+        const float p_i = vr.w;
+        const float h = P.h, h2 = P.h2, neg_m_spg = P.neg_m_spg;
+        const float mu2_m_lap = 2.0f * P.mu_m_lap;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 
-    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
-        for (uint32_t jj = b; jj < e; jj += kForceChunk) {
-            uint32_t mask = 0u;
+        UnionRows U;
+        if (plan_union(cstart, cc, P, U)) {
+            const bool any_slow = __any(slow);
+            auto walk = [&](auto slow_tag) {
+                constexpr bool AS = decltype(slow_tag)::value;
+                auto cand = [&](uint32_t j, float4 q, float4 w, auto self_tag) {
+                    float dx, dy, dz;
+                    const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                    if (d2 < h2 && (!decltype(self_tag)::value || j != s)) {
+                        float am = q.w;  // q.w * 1.0f
+                        if (AS && slow)
+                            am = q.w * (float)bucket_multiplicity(ci.x, ci.y, ci.z,
+                                                                  __float_as_uint(posB[j].w) & 0xFFFFu);
+                        force_pair(d2, dx, dy, dz, am, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);
+                    }
+                };
+                auto run = [&](uint32_t j0, uint32_t j1, auto self_tag) {
+                    const float4 *pq = posR + j0;  // wave-uniform: scalar loads
+                    const float4 *pw = velR + j0;
+                    uint32_t j = j0;
+                    for (; j + 4u <= j1; j += 4u, pq += 4, pw += 4) {
+                        const float4 q0 = pq[0], q1 = pq[1], q2 = pq[2], q3 = pq[3];
+                        const float4 w0 = pw[0], w1 = pw[1], w2 = pw[2], w3 = pw[3];
+                        cand(j, q0, w0, self_tag);
+                        cand(j + 1u, q1, w1, self_tag);
+                        cand(j + 2u, q2, w2, self_tag);
+                        cand(j + 3u, q3, w3, self_tag);
+                    }
+                    for (; j < j1; ++j, ++pq, ++pw)
+                        cand(j, pq[0], pw[0], self_tag);
+                };
+                for_each_union_row(cstart, U, P, [&](uint32_t b, uint32_t e) { split_self(b, e, L.first, run); });
+            };
+            if (any_slow)
+                walk(std::integral_constant<bool, true>{});
+            else
+                walk(std::integral_constant<bool, false>{});
+        } else {
+            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+                for (uint32_t jj = b; jj < e; jj += kForceChunk) {
+                    uint32_t mask = 0u;
 #pragma unroll
-            for (int k = 0; k < kForceChunk; ++k) {
-                const uint32_t j = jj + (uint32_t)k;
-                const float4 q = posR[j < e ? j : e - 1u];
-                float dx, dy, dz;
-                const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                const bool pass = (j < e) && (j != s) && (d2 < h2);
-                mask |= pass ? (1u << k) : 0u;
-            }
-            while (mask != 0u) {
-                const uint32_t k = (uint32_t)__builtin_ctz(mask);
-                mask &= mask - 1u;
-                const uint32_t j = jj + k;
-                const float4 q = posR[j];  // w = 0.5 / rho_j
-                const float4 w = velR[j];  // w = p_j
-                float dx, dy, dz;
-                const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                const float m = slow ? (float)bucket_multiplicity(ci.x, ci.y, ci.z,
-                                                                  __float_as_uint(posB[j].w) & 0xFFFFu)
-                                     : 1.0f;
-                // r = sqrt(d2) correctly rounded and hr = h - r exactly as
-                // the reference forms them: near the support edge h - r
-                // cancels and any error in r would be amplified.
-                const float r = sqrt_rn(d2);
-                const float rinv = __builtin_amdgcn_rcpf(r);
-                const float hr = h - r;
-                const float am = q.w * m;
-                const float cp = neg_m_spg * (p_i + w.w) * am * (hr * hr) * rinv;
-                const float cv = mu2_m_lap * hr * am;
-                fx = fmaf(cp, dx, fx);
-                fy = fmaf(cp, dy, fy);
-                fz = fmaf(cp, dz, fz);
-                fx = fmaf(cv, w.x - vi.x, fx);
-                fy = fmaf(cv, w.y - vi.y, fy);
-                fz = fmaf(cv, w.z - vi.z, fz);
-            }
+                    for (int k = 0; k < kForceChunk; ++k) {
+                        const uint32_t j = jj + (uint32_t)k;
+                        const float4 q = posR[j < e ? j : e - 1u];
+                        float dx, dy, dz;
+                        const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                        const bool pass = (j < e) && (j != s) && (d2 < h2);
+                        mask |= pass ? (1u << k) : 0u;
+                    }
+                    while (mask != 0u) {
+                        const uint32_t k = (uint32_t)__builtin_ctz(mask);
+                        mask &= mask - 1u;
+                        const uint32_t j = jj + k;
+                        const float4 q = posR[j];  // w = 0.5 / rho_j
+                        const float4 w = velR[j];  // w = p_j
+                        float dx, dy, dz;
+                        const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                        float am = q.w;
+                        if (slow)
+                            am = q.w * (float)bucket_multiplicity(ci.x, ci.y, ci.z,
+                                                                  __float_as_uint(posB[j].w) & 0xFFFFu);
+                        force_pair(d2, dx, dy, dz, am, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);
+                    }
+                }
+            });
         }
-    });
-    if (WRITE_FORCE)
-        force_out[s] = make_float4(fx, fy, fz, 0.0f);
-    integrate_and_walls(pi, vi, fx, fy, fz, rho_i, P);
-    if (NEXT_KEYS) {
-        Classified c = classify(pi.x, pi.y, pi.z, P);
-        pi.w = __uint_as_float(c.h16);
-        cellcA[s] = c.cellc;
-        rankA[s] = atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
-    } else {
-        pi.w = 0.0f;
+        if (!active)
+            return;
+        if (WRITE_FORCE)
+            force_out[s] = make_float4(fx, fy, fz, 0.0f);
+        integrate_and_walls(pi, vi, fx, fy, fz, rhoB[s], P);
+        if (NEXT_KEYS) {
+            Classified c = classify(pi.x, pi.y, pi.z, P);
+            pi.w = __uint_as_float(c.h16);
+            cellcA[s] = c.cellc;
+            rankA[s] = atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
+        } else {
+            pi.w = 0.0f;
+        }
+        posA[s] = pi;
+        velA[s] = vi;
     }
-    posA[s] = pi;
-    velA[s] = vi;
 }
 
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,

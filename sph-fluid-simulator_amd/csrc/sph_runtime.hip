@@ -497,7 +497,7 @@ int update_particles(sph_particle_v1 *particles, float *transforms, size_t n_,
     hipStream_t st = S.stream;
     ParticleArrays a = w.particles();
     CellArrays c = w.cells();
-    SceneSet set = w.scene_set();
+    const SceneSet set = w.scene_set();
     launch_density(st, a, c, set, nullptr);
     launch_force_integrate(st, a, c, set, (float4 *)w.force.ptr, /*next_keys=*/false);
     CellArrays c16 = cells16(S);

@@ -30,6 +30,7 @@ SPH_E_HIP = -4
 SPH_E_LIMIT = -5
 SPH_E_STATE = -6
 SPH_SCENE_KEEP_FORCE = 1
+SPH_SCENE_LANE_WALK = 2
 
 # struct Particle (src/Particle.h:4-10): 60 bytes.
 PARTICLE_DTYPE = np.dtype(
@@ -226,7 +227,7 @@ class Context:
     """Resident single scene: SPHSystem::update without per-step PCIe."""
 
     def __init__(self, n: int, s: Settings, box_: Box, dt: float, cell_scale: float = 1.0,
-                 keep_force: bool = False):
+                 keep_force: bool = False, lane_walk: bool = False):
         self._lib = load()
         d = SceneDesc()
         d.n = n
@@ -234,7 +235,7 @@ class Context:
         d.settings = s
         d.box = box_
         d.cell_scale = cell_scale
-        d.flags = SPH_SCENE_KEEP_FORCE if keep_force else 0
+        d.flags = (SPH_SCENE_KEEP_FORCE if keep_force else 0) | (SPH_SCENE_LANE_WALK if lane_walk else 0)
         h = c_void_p()
         _check(self._lib.sph_hip_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -297,14 +298,14 @@ class Context:
 
 
 def scene_desc(n: int, s: Settings, box_: Box, dt: float, cell_scale: float = 1.0,
-               keep_force: bool = False) -> SceneDesc:
+               keep_force: bool = False, lane_walk: bool = False) -> SceneDesc:
     d = SceneDesc()
     d.n = n
     d.dt = dt
     d.settings = s
     d.box = box_
     d.cell_scale = cell_scale
-    d.flags = SPH_SCENE_KEEP_FORCE if keep_force else 0
+    d.flags = (SPH_SCENE_KEEP_FORCE if keep_force else 0) | (SPH_SCENE_LANE_WALK if lane_walk else 0)
     return d
 
 

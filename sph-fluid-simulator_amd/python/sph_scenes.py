@@ -152,3 +152,18 @@ def batched_workload(nscenes: int = 256, seed: int = 1024, dt: float = 0.002):
         scenes.append(Scene(f"batched-{k}-{n}", pos, np.zeros_like(pos), mass, 1000.0, gas, mu, h, -9.8, 0.2,
                             dt, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), steps=500))
     return scenes
+
+
+def blobs(nblobs: int = 12, per_blob: int = 2000, h: float = 0.05, sigma_h: float = 0.5, seed: int = 31) -> Scene:
+    """Gaussian clumps (sigma = sigma_h h) of per_blob particles each, at
+    random centres in [-0.6, 0.6]^3 inside the box [-1,1]^3: the clumped
+    state the dam-break workloads reach, with cells holding hundreds of
+    particles and waves that span one to a few cells."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-0.6, 0.6, size=(nblobs, 3))
+    pos = (np.repeat(centres, per_blob, axis=0) + rng.normal(0.0, sigma_h * h, size=(nblobs * per_blob, 3)))
+    pos = np.clip(pos, -0.9, 0.9).astype(F32)
+    vel = rng.normal(0.0, 0.05, size=pos.shape).astype(F32)
+    mass = float(F32(0.02 * (h / 0.15) ** 3))
+    return Scene(f"blobs-{nblobs}x{per_blob}", pos, vel, mass, 1000.0, 1.0, 1.04, h, -9.8, 0.2,
+                 0.001, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), steps=200)

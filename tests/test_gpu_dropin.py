@@ -238,3 +238,12 @@ def test_dropin_repeatable(sph):
     a1, x1 = sph.update_particles(inp, s, sc.dt, b)
     a2, x2 = sph.update_particles(inp, s, sc.dt, b)
     assert a1.tobytes() == a2.tobytes() and x1.tobytes() == x2.tobytes()
+
+
+def test_dropin_clumped_blobs(sph, orc32):
+    """Cells of hundreds of particles: most waves take the wave-uniform
+    candidate walk, some span several cells."""
+    sc = sph_scenes.blobs()
+    ratio = check_dropin(sph, orc32, sc.pos, sc.vel, sc.settings_args(), sc.dt, sc.box_lo, sc.box_hi,
+                         sc.restitution, sc.offset, "blobs")
+    assert ratio <= 1.0
