@@ -19,6 +19,14 @@ The JSON line also carries
   cpu_baseline  the oracle (CPU restatement of the reference's
                 multithreaded update()) timed on this host over a few steps
                 of the same workload
+  batched       the batched workload: 256 independent scenes (1k-64k
+                particles each, log-uniform) advanced together for 500
+                steps after the autotuner picked its settings; scene-steps/s,
+                per-step p50/p99, and the same scenes stepped one after
+                another on the plain single-scene path for comparison
+  serving       the serving loop under Poisson request streams (2k-500k
+                particles, 10-1000 steps, dam-break or droplet) at several
+                rates: per-step p50/p99 and aggregate particle-updates/s
 """
 import argparse
 import json
@@ -34,6 +42,7 @@ import numpy as np  # noqa: E402
 
 import sph_amd  # noqa: E402
 import sph_scenes  # noqa: E402
+import sph_serving  # noqa: E402
 
 METRIC = "particle-updates/s at 1M (1 GPU, % HBM roofline); batched scene-steps/s + p99"
 HBM_PEAK_GBS = 8000.0
@@ -92,6 +101,70 @@ def load_traffic(kernel):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024):
+    scenes = sph_scenes.batched_workload(nscenes, seed)
+    descs = [sph_serving.scene_desc(sc) for sc in scenes]
+    total = sum(sc.n for sc in scenes)
+    log(f"batched: {nscenes} scenes, {total} particles, {steps} steps")
+    with sph_amd.Batch(descs) as b:
+        for k, sc in enumerate(scenes):
+            b.upload(k, sc.pos, sc.vel)
+        tune = b.autotune(tune_steps)
+        log(f"autotune: {tune}")
+        b.step_timed(2)  # graph capture
+        for k, sc in enumerate(scenes):  # the timed job starts from the initial states
+            b.upload(k, sc.pos, sc.vel)
+        b.sync()
+        t0 = time.perf_counter()
+        ms = b.step_timed(steps)
+        wall = time.perf_counter() - t0
+    batched_rate = nscenes * steps / wall
+    log(f"batched: {wall:.3f} s, {batched_rate:.0f} scene-steps/s, step p50 {np.percentile(ms, 50):.3f} "
+        f"p99 {np.percentile(ms, 99):.3f} ms")
+    # the same scenes, one after another, each on its own single-scene context
+    seq = 0.0
+    for sc, d in zip(scenes, descs):
+        with sph_amd.Context(sc.n, d.settings, d.box, sc.dt) as ctx:
+            ctx.upload(sc.pos, sc.vel)
+            ctx.step(1)  # graph capture
+            ctx.upload(sc.pos, sc.vel)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            ctx.sync()
+            seq += time.perf_counter() - t0
+    seq_rate = nscenes * steps / seq
+    log(f"sequential single-scene stepping: {seq:.3f} s, {seq_rate:.0f} scene-steps/s")
+    return {
+        "workload": f"{nscenes} scenes, particles log-uniform in [1k, 64k] (total {total}), h = 2.1 spacing, "
+                    f"viscosity U[0.5,2], gas constant U[0.5,3], {steps} steps, one launch per pass, graphed",
+        "scene_steps_per_s": batched_rate,
+        "particle_updates_per_s": total * steps / wall,
+        "step_ms_p50": float(np.percentile(ms, 50)),
+        "step_ms_p99": float(np.percentile(ms, 99)),
+        "sequential_scene_steps_per_s": seq_rate,
+        "speedup_vs_sequential": batched_rate / seq_rate,
+        "autotune": tune,
+    }
+
+
+def serving_leg(rates, window, budget, pool_size=12):
+    pool = sph_serving.make_pool(pool_size)
+    out = []
+    for r in rates:
+        res = sph_serving.run_stream(rate=r, duration=window, pool=pool, memory_budget=budget)
+        log(f"serving at {r} req/s: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
+                                                   for k, v in res.items()))
+        out.append(res)
+    return {
+        "requests": "Poisson arrivals; particles log-uniform [2k, 500k] (pool of "
+                    f"{pool_size} pre-generated scenes), steps U[10, 1000], dam-break or droplet",
+        "memory_budget_bytes": budget,
+        "window_s_per_rate": window,
+        "rates": out,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +175,12 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event pass")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run the copy kernel over this many bytes (HBM counter calibration)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes leg")
+    ap.add_argument("--batched-steps", type=int, default=500)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serving-rates", default="50,500,2000", help="request rates (per second) to replay")
+    ap.add_argument("--serving-window", type=float, default=5.0, help="seconds of arrivals per rate")
+    ap.add_argument("--serving-budget", type=int, default=2 << 30, help="device bytes for admitted scenes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -159,6 +238,12 @@ def main():
         cpu = cpu_baseline(sc)
         log(f"cpu baseline: {cpu['value'] / 1e6:.3f} M particle-updates/s")
 
+    batched = None if args.no_batched else batched_leg(steps=args.batched_steps)
+    serving = None
+    if not args.no_serving:
+        rates = [float(x) for x in args.serving_rates.split(",") if x]
+        serving = serving_leg(rates, args.serving_window, args.serving_budget)
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -180,6 +265,8 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

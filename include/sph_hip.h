@@ -190,6 +190,92 @@ int sph_batch_set_graph(sph_batch *b, int enable);
 uint32_t sph_batch_scene_count(const sph_batch *b);
 int sph_batch_profile(sph_batch *b, int nsteps, double *ms, int max_k,
                       int *nk, char *names_buf, size_t names_cap);
+/* sph_batch_step with the GPU time of every step: ms[k] (may be NULL) for
+ * step k.  Synchronises. */
+int sph_batch_step_timed(sph_batch *b, int nsteps, float *ms);
+
+/* ---- autotuner --------------------------------------------------------------
+ * Times trial_steps steps from the current state for each stencil block
+ * size (64, 128, 256 threads) and walk mode (wave-uniform walk allowed, or
+ * per-lane only), restoring the state before every trial and after the
+ * last, and keeps the fastest for the following steps.  Neither knob
+ * changes any result bit.  trial_ms[k] is the mean step time of trial k,
+ * k = 3 * lane_walk + (0, 1, 2 for 64, 128, 256 threads).  The reference
+ * has no counterpart (its thread count is hardware_concurrency,
+ * src/sph.cpp:199). */
+typedef struct sph_tune_v1 {
+    int32_t stencil_block;       /* chosen threads per block */
+    int32_t lane_walk;           /* 1: per-lane walk chosen */
+    int32_t trials;
+    int32_t _pad;
+    double ms_per_step;          /* mean step time of the chosen setting */
+    double default_ms_per_step;  /* same for 128 threads, wave-uniform walk */
+    double trial_ms[6];
+} sph_tune_v1;
+
+int sph_batch_autotune(sph_batch *b, int trial_steps, sph_tune_v1 *out);
+int sph_hip_autotune(sph_ctx *ctx, int trial_steps, sph_tune_v1 *out);
+
+/* ---- serving loop -----------------------------------------------------------
+ * Continuous batching of scene-advance requests.  No counterpart in the
+ * reference (its only caller advances one system per frame,
+ * src/SPHSystem.cpp:110-117).  submit() queues a request (the initial state
+ * is copied, so the caller's arrays may be reused at once); tick() runs one
+ * scheduling round: it admits queued requests first-come first-served while
+ * their device memory fits the budget and a scene slot is free, advances
+ * every admitted scene by the same number of steps (up to
+ * max_steps_per_tick, never past any request's end), and retires finished
+ * requests, whose final state waits in host memory for fetch().  Admitted
+ * scenes share every kernel launch; each result is bit-identical to the
+ * same scene stepped alone with sph_hip_*.  A request that can never fit
+ * the budget is refused at submit with SPH_E_LIMIT. */
+typedef struct sph_server sph_server;
+
+typedef struct sph_server_config_v1 {
+    uint64_t memory_budget;      /* device bytes for admitted scenes; 0 = 3/4 of free memory */
+    uint32_t max_scenes;         /* admitted scenes at once; 0 = 1024 */
+    uint32_t max_steps_per_tick; /* batch steps per tick; 0 = 8, at most 256 */
+    uint32_t flags;              /* SPH_SERVER_* bits, others must be 0 */
+} sph_server_config_v1;
+
+/* Step with stream launches instead of one captured graph per step. */
+#define SPH_SERVER_NO_GRAPH 1u
+
+#define SPH_REQ_QUEUED 0
+#define SPH_REQ_ACTIVE 1
+#define SPH_REQ_DONE 2
+#define SPH_REQ_FETCHED 3
+
+typedef struct sph_server_stats_v1 {
+    uint64_t submitted, admitted, completed, queued, active;
+    uint64_t steps;              /* batch steps run */
+    uint64_t used_bytes, budget_bytes;
+    double particle_steps;       /* sum over batch steps of admitted particles */
+    double scene_steps;          /* sum over batch steps of admitted scenes */
+    double busy_ms;              /* GPU time of all batch steps */
+    double wall_s;               /* seconds since create */
+    double step_ms_p50, step_ms_p99, step_ms_max;  /* per batch step */
+} sph_server_stats_v1;
+
+int sph_server_create(const sph_server_config_v1 *cfg, sph_server **out);
+int sph_server_destroy(sph_server *s);
+int sph_server_submit(sph_server *s, const sph_scene_desc_v1 *desc,
+                      const float *pos, const float *vel, uint32_t steps,
+                      uint64_t *request_id);
+/* *ran_steps (may be NULL) receives the batch steps run in this round. */
+int sph_server_tick(sph_server *s, uint32_t *ran_steps);
+/* *state: SPH_REQ_*; *done_steps (may be NULL): steps run so far; times
+ * (may be NULL): seconds since create of submit, admission and completion. */
+int sph_server_status(const sph_server *s, uint64_t request_id, int *state,
+                      uint32_t *done_steps, double times[3]);
+/* Final state of a finished request (n x 3 positions and velocities, n
+ * densities, pressures and ids, any may be NULL) in device order; frees it. */
+int sph_server_fetch(sph_server *s, uint64_t request_id, float *pos,
+                     float *vel, float *rho, float *pressure, uint32_t *ids);
+int sph_server_stats(const sph_server *s, sph_server_stats_v1 *out);
+/* GPU time of every batch step so far, oldest first (up to cap values). */
+int sph_server_step_times(const sph_server *s, float *ms, size_t cap,
+                          size_t *n);
 
 /* ---- measurement ---------------------------------------------------------
  * Runs nsteps un-graphed steps with HIP events around every kernel on the

@@ -19,6 +19,8 @@ struct BatchScene {
 
 int validate_desc(const sph_scene_desc_v1 &d);
 void scene_grid(StepParams &P, const sph_scene_desc_v1 &d, const float *pos, uint32_t n);
+// A scene entry for desc (validated): constants, flags; no slot range yet.
+int make_scene(const sph_scene_desc_v1 &d, BatchScene &out);
 
 struct Batch {
     hipStream_t stream = nullptr;
@@ -27,9 +29,11 @@ struct Batch {
     uint32_t total_n = 0;
     bool layout_dirty = true, keys_dirty = true;
     bool use_graph = true, keep_force = false;
+    uint64_t max_capacity = 0;  // particle capacity relayout may grow to (0: unbounded)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     DevBuf aos, xf;  // download staging
+    std::vector<hipEvent_t> events;
 
     Batch() = default;
     Batch(const Batch &) = delete;
@@ -37,11 +41,27 @@ struct Batch {
     ~Batch();
 
     int init(const sph_scene_desc_v1 *descs, uint32_t k);
+    // A batch with no scenes yet (the serving loop sets them with relayout).
+    int init_empty(bool keep_force_);
+    // Replaces the scene list with `next`: next[k] continues old scene
+    // from[k] (its particle state moves with device copies, in the same
+    // order, so it steps on bit-identically) or, for from[k] < 0, is a new
+    // scene to upload before stepping.  Densities, pressures and forces of
+    // continued scenes are not carried over (the next step recomputes them).
+    int relayout(std::vector<BatchScene> next, const std::vector<int> &from);
     int upload(uint32_t s, const float *pos, const float *vel);
     // Lays out cells and the scene table and computes keys when needed.
     int prepare();
     void enqueue(StepTimer *timer);
     int step(int nsteps);
+    // step() with an event after every step; ms[k] (may be NULL) receives
+    // the GPU time of step k.  Synchronises.
+    int step_timed(int nsteps, float *ms);
+    // Times trial_steps steps from the current state for every stencil
+    // block size (64, 128, 256) and walk mode (wave-uniform walk allowed or
+    // per-lane only), restoring the state before each trial and after the
+    // last, and keeps the fastest.  Results do not depend on either knob.
+    int autotune(int trial_steps, sph_tune_v1 *out);
     int sync();
     int download(uint32_t s, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids);
     int download_particles(uint32_t s, sph_particle_v1 *out, float *transforms);

@@ -88,10 +88,24 @@ Batch::~Batch()
 {
     if (stream)
         (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : events)
+        (void)hipEventDestroy(e);
     drop_graph();
     w.release_all();
     if (stream)
         (void)hipStreamDestroy(stream);
+}
+
+int make_scene(const sph_scene_desc_v1 &d, BatchScene &out)
+{
+    int rc;
+    if ((rc = validate_desc(d)))
+        return rc;
+    out = BatchScene();
+    out.desc = d;
+    out.P = make_params(d.settings, d.box, d.dt);
+    out.P.lane_walk = (d.flags & SPH_SCENE_LANE_WALK) ? 1u : 0u;
+    return SPH_OK;
 }
 
 int Batch::init(const sph_scene_desc_v1 *descs, uint32_t k)
@@ -102,21 +116,30 @@ int Batch::init(const sph_scene_desc_v1 *descs, uint32_t k)
     uint64_t total = 0;
     scenes.clear();
     for (uint32_t s = 0; s < k; ++s) {
-        if ((rc = validate_desc(descs[s])))
-            return rc;
         BatchScene b;
-        b.desc = descs[s];
-        b.P = make_params(descs[s].settings, descs[s].box, descs[s].dt);
+        if ((rc = make_scene(descs[s], b)))
+            return rc;
         b.P.p_begin = (uint32_t)total;
         total += descs[s].n;
         if (total > kMaxParticles)
             return fail(SPH_E_LIMIT, "batch: %llu particles exceed the 2^30 limit", (unsigned long long)total);
         b.P.p_end = (uint32_t)total;
-        b.P.lane_walk = (descs[s].flags & SPH_SCENE_LANE_WALK) ? 1u : 0u;
         keep_force = keep_force || (descs[s].flags & SPH_SCENE_KEEP_FORCE);
         scenes.push_back(b);
     }
     total_n = (uint32_t)total;
+    if ((rc = init_empty(keep_force)))
+        return rc;
+    if ((rc = w.reserve_particles(total_n)) ||
+        (keep_force && (rc = w.force.reserve(sizeof(float4) * (size_t)total_n))))
+        return rc;
+    return SPH_OK;
+}
+
+int Batch::init_empty(bool keep_force_)
+{
+    int rc;
+    keep_force = keep_force_;
     if ((rc = ensure_device()))
         return rc;
     hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
@@ -124,9 +147,86 @@ int Batch::init(const sph_scene_desc_v1 *descs, uint32_t k)
         stream = nullptr;
         return hip_fail(e, "hipStreamCreate");
     }
-    if ((rc = w.reserve_particles(total_n)) ||
-        (keep_force && (rc = w.force.reserve(sizeof(float4) * (size_t)total_n))))
-        return rc;
+    return SPH_OK;
+}
+
+namespace {
+void swap_buf(DevBuf &x, DevBuf &y)
+{
+    std::swap(x.ptr, y.ptr);
+    std::swap(x.bytes, y.bytes);
+}
+}  // namespace
+
+int Batch::relayout(std::vector<BatchScene> next, const std::vector<int> &from)
+{
+    int rc;
+    if (next.size() != from.size())
+        return fail(SPH_E_INVALID, "relayout: size mismatch");
+    uint64_t total = 0;
+    for (size_t k = 0; k < next.size(); ++k) {
+        if (from[k] >= (int)scenes.size())
+            return fail(SPH_E_INVALID, "relayout: bad source scene");
+        next[k].P.p_begin = (uint32_t)total;
+        total += next[k].desc.n;
+        if (total > kMaxParticles)
+            return fail(SPH_E_LIMIT, "relayout: %llu particles exceed the 2^30 limit", (unsigned long long)total);
+        next[k].P.p_end = (uint32_t)total;
+        next[k].uploaded = from[k] >= 0 && scenes[from[k]].uploaded;
+    }
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "relayout");
+    drop_graph();
+    // Destination of the moved state: the B arrays (scratch between steps)
+    // when they are large enough, else fresh buffers with head room.
+    const uint64_t cap = w.posA.bytes / sizeof(float4);
+    const bool grow = total > cap;
+    DevBuf np, nv;
+    float4 *dp = (float4 *)w.posB.ptr, *dv = (float4 *)w.velB.ptr;
+    if (grow) {
+        uint64_t want = std::max<uint64_t>(total, cap + cap / 2);
+        if (max_capacity)
+            want = std::min<uint64_t>(want, std::max<uint64_t>(total, max_capacity));
+        want = std::min<uint64_t>(want, kMaxParticles);
+        if ((rc = np.reserve(sizeof(float4) * want)) || (rc = nv.reserve(sizeof(float4) * want)))
+            return rc;
+        dp = (float4 *)np.ptr;
+        dv = (float4 *)nv.ptr;
+    }
+    for (size_t k = 0; k < next.size(); ++k) {
+        if (from[k] < 0 || !next[k].uploaded)
+            continue;
+        const BatchScene &o = scenes[from[k]];
+        const size_t bytes = sizeof(float4) * (size_t)o.desc.n;
+        e = hipMemcpyAsync(dp + next[k].P.p_begin, (float4 *)w.posA.ptr + o.P.p_begin, bytes,
+                           hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dv + next[k].P.p_begin, (float4 *)w.velA.ptr + o.P.p_begin, bytes,
+                               hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess)
+            return hip_fail(e, "relayout copy");
+    }
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "relayout copy");
+    if (grow) {
+        swap_buf(w.posA, np);
+        swap_buf(w.velA, nv);
+        np.release();
+        nv.release();
+        const uint32_t want = (uint32_t)(w.posA.bytes / sizeof(float4));
+        if ((rc = w.reserve_particles(want)) ||
+            (keep_force && (rc = w.force.reserve(sizeof(float4) * (size_t)want))))
+            return rc;
+    } else {
+        swap_buf(w.posA, w.posB);
+        swap_buf(w.velA, w.velB);
+    }
+    scenes = std::move(next);
+    total_n = (uint32_t)total;
+    layout_dirty = true;
+    keys_dirty = true;
     return SPH_OK;
 }
 
@@ -216,6 +316,8 @@ int Batch::step(int nsteps)
     int rc;
     if (nsteps < 0)
         return fail(SPH_E_INVALID, "step: negative step count");
+    if (scenes.empty())
+        return SPH_OK;
     if ((rc = prepare()))
         return rc;
     if (use_graph) {
@@ -244,6 +346,108 @@ int Batch::step(int nsteps)
         if (e != hipSuccess)
             return hip_fail(e, "step launch");
     }
+    return SPH_OK;
+}
+
+int Batch::step_timed(int nsteps, float *ms)
+{
+    int rc;
+    if (nsteps < 0)
+        return fail(SPH_E_INVALID, "step: negative step count");
+    if (scenes.empty() || nsteps == 0)
+        return SPH_OK;
+    if ((rc = prepare()))
+        return rc;
+    while (events.size() < (size_t)nsteps + 1) {
+        hipEvent_t e;
+        hipError_t err = hipEventCreate(&e);
+        if (err != hipSuccess)
+            return hip_fail(err, "hipEventCreate");
+        events.push_back(e);
+    }
+    hipError_t e = hipEventRecord(events[0], stream);
+    for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+        if ((rc = step(1)))
+            return rc;
+        e = hipEventRecord(events[s + 1], stream);
+    }
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "timed step");
+    for (int s = 0; ms && s < nsteps; ++s) {
+        ms[s] = 0.0f;
+        (void)hipEventElapsedTime(&ms[s], events[s], events[s + 1]);
+    }
+    return SPH_OK;
+}
+
+int Batch::autotune(int trial_steps, sph_tune_v1 *out)
+{
+    int rc;
+    if (trial_steps <= 0 || !out)
+        return fail(SPH_E_INVALID, "autotune: bad arguments");
+    if (scenes.empty())
+        return fail(SPH_E_STATE, "autotune: no scenes");
+    if ((rc = prepare()))
+        return rc;
+    std::memset(out, 0, sizeof *out);
+    // The state every trial starts from (and is restored to at the end).
+    DevBuf sp, sv;
+    const size_t bytes = sizeof(float4) * (size_t)total_n;
+    if ((rc = sp.reserve(bytes)) || (rc = sv.reserve(bytes)))
+        return rc;
+    hipError_t e = hipMemcpyAsync(sp.ptr, w.posA.ptr, bytes, hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sv.ptr, w.velA.ptr, bytes, hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "autotune snapshot");
+    auto restore = [&]() -> int {
+        hipError_t err = hipMemcpyAsync(w.posA.ptr, sp.ptr, bytes, hipMemcpyDeviceToDevice, stream);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(w.velA.ptr, sv.ptr, bytes, hipMemcpyDeviceToDevice, stream);
+        if (err != hipSuccess)
+            return hip_fail(err, "autotune restore");
+        keys_dirty = true;
+        return SPH_OK;
+    };
+    auto apply = [&](int block, int lane) {
+        w.stencil_block = block;
+        for (BatchScene &b : scenes)
+            b.P.lane_walk = (lane || (b.desc.flags & SPH_SCENE_LANE_WALK)) ? 1u : 0u;
+        layout_dirty = true;  // re-uploads the scene table
+    };
+    const int blocks[3] = {64, 128, 256};
+    std::vector<float> ms((size_t)trial_steps);
+    double best = -1.0;
+    int k = 0;
+    for (int lane = 0; lane < 2; ++lane)
+        for (int bi = 0; bi < 3; ++bi, ++k) {
+            apply(blocks[bi], lane);
+            if ((rc = restore()) || (rc = step_timed(1, nullptr)) || (rc = restore()) ||
+                (rc = step_timed(trial_steps, ms.data())))
+                return rc;
+            double sum = 0.0;
+            for (float v : ms)
+                sum += v;
+            const double mean = sum / trial_steps;
+            out->trial_ms[k] = mean;
+            if (blocks[bi] == kStencilBlock && lane == 0)
+                out->default_ms_per_step = mean;
+            if (best < 0.0 || mean < best) {
+                best = mean;
+                out->stencil_block = blocks[bi];
+                out->lane_walk = lane;
+            }
+        }
+    out->ms_per_step = best;
+    out->trials = k;
+    apply(out->stencil_block, out->lane_walk);
+    if ((rc = restore()))
+        return rc;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "autotune");
     return SPH_OK;
 }
 
@@ -536,6 +740,27 @@ int sph_batch_profile(sph_batch *b, int nsteps, double *ms, int max_k, int *nk, 
     if (!b)
         return fail(SPH_E_INVALID, "batch profile: NULL batch");
     return b->b.profile(nsteps, ms, max_k, nk, names_buf, names_cap);
+}
+
+int sph_batch_step_timed(sph_batch *b, int nsteps, float *ms)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch step_timed: NULL batch");
+    return b->b.step_timed(nsteps, ms);
+}
+
+int sph_batch_autotune(sph_batch *b, int trial_steps, sph_tune_v1 *out)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch autotune: NULL batch");
+    return b->b.autotune(trial_steps, out);
+}
+
+int sph_hip_autotune(sph_ctx *c, int trial_steps, sph_tune_v1 *out)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "autotune: NULL context");
+    return c->b.autotune(trial_steps, out);
 }
 
 }  // extern "C"

@@ -31,7 +31,7 @@
 namespace sph {
 
 constexpr int kBlock = 256;           // threads per block, particle kernels
-constexpr int kStencilBlock = 128;    // threads per block, density / force
+constexpr int kStencilBlock = 128;    // default threads per block, density / force
 constexpr int kForceChunk = 16;       // candidates filtered per bitmask in the force pass
 constexpr int kScanItems = 16;        // cells per thread in the scan
 constexpr uint32_t kScanChunk = kBlock * kScanItems;  // cells per scan block
@@ -89,10 +89,12 @@ void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c
 // Sorts the A indices of every listed big cell (deterministic in-cell order).
 void launch_sort_big_cells(hipStream_t st, const ParticleArrays &a, const CellArrays &c);
 void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S);
+// `block`: threads per block of the stencil passes, 64, 128 or 256 (the
+// dump modes always use kStencilBlock); results do not depend on it.
 void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                    const NeighborDump *dump);
+                    const NeighborDump *dump, int block = kStencilBlock);
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                            const SceneSet &S, float4 *force_out, bool next_keys);
+                            const SceneSet &S, float4 *force_out, bool next_keys, int block = kStencilBlock);
 // Drop-in output (single scene): order slots by the pre-step 16-bit bucket
 // (ties by id), and write the reference AoS layout plus transforms.
 void launch_h16_rank(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n);

@@ -624,8 +624,8 @@ __device__ inline void for_each_union_row(const uint32_t *__restrict__ cstart, c
 // MODE 0: density + pressure.  MODE 1: also the neighbour count per
 // particle.  MODE 2: also the neighbour ids (multiplicity-repeated).
 // Writes rho, posR = (x, 0.5 / rho) and velR = (v, p) for the force pass.
-template <int MODE>
-__global__ __launch_bounds__(kStencilBlock) void k_density(
+template <int MODE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_density(
     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
     const uint32_t *__restrict__ cellc, const uint32_t *__restrict__ cstart,
     float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, SceneSet S,
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(kStencilBlock) void k_density(
 {
     Lane L;
     bool active;
-    if (!wave_lanes<kStencilBlock>(S, L, active))
+    if (!wave_lanes<BLOCK>(S, L, active))
         return;
     {
         const StepParams &P = *L.P;
@@ -732,20 +732,29 @@ __global__ __launch_bounds__(kStencilBlock) void k_density(
     }
 }
 
+template <int MODE, int BLOCK>
+static void launch_density_block(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
+                                 const SceneSet &S, const NeighborDump &d)
+{
+    hipLaunchKernelGGL((k_density<MODE, BLOCK>), dim3(blocks_for_waves(S.nwaves, BLOCK)), dim3(BLOCK), 0, st,
+                       a.posB, a.velB, a.cellcB, c.cstart, a.rhoB, a.posR, a.velR, S, d);
+}
+
 void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                    const NeighborDump *dump)
+                    const NeighborDump *dump, int block)
 {
     NeighborDump d = dump ? *dump : NeighborDump{nullptr, nullptr, nullptr};
-    dim3 g(blocks_for_waves(S.nwaves, kStencilBlock)), b(kStencilBlock);
-    if (!dump)
-        hipLaunchKernelGGL(k_density<0>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.posR, a.velR, S, d);
-    else if (!dump->out)
-        hipLaunchKernelGGL(k_density<1>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.posR, a.velR, S, d);
-    else
-        hipLaunchKernelGGL(k_density<2>, g, b, 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB,
-                           a.posR, a.velR, S, d);
+    if (dump)
+        return dump->out ? launch_density_block<2, kStencilBlock>(st, a, c, S, d)
+                         : launch_density_block<1, kStencilBlock>(st, a, c, S, d);
+    switch (block) {
+    case 64:
+        return launch_density_block<0, 64>(st, a, c, S, d);
+    case 256:
+        return launch_density_block<0, 256>(st, a, c, S, d);
+    default:
+        return launch_density_block<0, kStencilBlock>(st, a, c, S, d);
+    }
 }
 
 // ----------------------------------------------------- force + integrate
@@ -827,8 +836,8 @@ __device__ inline void force_pair(float d2, float dx, float dy, float dz, float 
 //  * per lane: candidates filtered kForceChunk at a time into a per-lane
 //    bitmask, the terms then run only over the set bits, so a wave pays for
 //    the most neighbours any of its lanes has in the chunk.
-template <bool WRITE_FORCE, bool NEXT_KEYS>
-__global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
+template <bool WRITE_FORCE, bool NEXT_KEYS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_force_integrate(
     const float4 *__restrict__ posB, const float4 *__restrict__ velB,
     const float4 *__restrict__ posR, const float4 *__restrict__ velR,
     const uint32_t *__restrict__ cellcB, const uint32_t *__restrict__ cstart,
@@ -838,7 +847,7 @@ __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
 {
     Lane L;
     bool active;
-    if (!wave_lanes<kStencilBlock>(S, L, active))
+    if (!wave_lanes<BLOCK>(S, L, active))
         return;
     {
         const StepParams &P = *L.P;
@@ -939,13 +948,14 @@ __global__ __launch_bounds__(kStencilBlock) void k_force_integrate(
     }
 }
 
-void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                            const SceneSet &S, float4 *force_out, bool next_keys)
+template <int BLOCK>
+static void launch_force_block(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
+                               float4 *force_out, bool next_keys)
 {
-    dim3 g(blocks_for_waves(S.nwaves, kStencilBlock)), b(kStencilBlock);
-#define SPH_LAUNCH_FI(WF, NK)                                                                   \
-    hipLaunchKernelGGL((k_force_integrate<WF, NK>), g, b, 0, st, a.posB, a.velB, a.posR, a.velR, \
-                       a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,  \
+    dim3 g(blocks_for_waves(S.nwaves, BLOCK)), b(BLOCK);
+#define SPH_LAUNCH_FI(WF, NK)                                                                          \
+    hipLaunchKernelGGL((k_force_integrate<WF, NK, BLOCK>), g, b, 0, st, a.posB, a.velB, a.posR, a.velR, \
+                       a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,         \
                        force_out, S)
     if (force_out && next_keys)
         SPH_LAUNCH_FI(true, true);
@@ -956,6 +966,19 @@ void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellA
     else
         SPH_LAUNCH_FI(false, false);
 #undef SPH_LAUNCH_FI
+}
+
+void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
+                            const SceneSet &S, float4 *force_out, bool next_keys, int block)
+{
+    switch (block) {
+    case 64:
+        return launch_force_block<64>(st, a, c, S, force_out, next_keys);
+    case 256:
+        return launch_force_block<256>(st, a, c, S, force_out, next_keys);
+    default:
+        return launch_force_block<kStencilBlock>(st, a, c, S, force_out, next_keys);
+    }
 }
 
 // ------------------------------------------------------- drop-in output

@@ -46,6 +46,7 @@ struct Workspace {
     uint32_t ncells_padded = 0;
     uint32_t big_cap = 0;
     uint32_t nwaves = 0;
+    int stencil_block = kStencilBlock;  // threads per block of the density and force passes
     int reserve_particles(uint32_t n);
     int reserve_cells(uint64_t ncells);
     // Uploads the scene table (p_begin/p_end/cell_base filled in) and builds

@@ -341,10 +341,10 @@ void enqueue_step(hipStream_t st, Workspace &w, float4 *force_out, bool next_key
     launch_reorder(st, a, c, S);
     if (timer)
         timer->mark(st, "reorder");
-    launch_density(st, a, c, S, nullptr);
+    launch_density(st, a, c, S, nullptr, w.stencil_block);
     if (timer)
         timer->mark(st, "density");
-    launch_force_integrate(st, a, c, S, force_out, next_keys);
+    launch_force_integrate(st, a, c, S, force_out, next_keys, w.stencil_block);
     if (timer)
         timer->mark(st, "force_integrate");
 }

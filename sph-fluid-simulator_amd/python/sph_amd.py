@@ -31,6 +31,8 @@ SPH_E_LIMIT = -5
 SPH_E_STATE = -6
 SPH_SCENE_KEEP_FORCE = 1
 SPH_SCENE_LANE_WALK = 2
+SPH_SERVER_NO_GRAPH = 1
+SPH_REQ_QUEUED, SPH_REQ_ACTIVE, SPH_REQ_DONE, SPH_REQ_FETCHED = 0, 1, 2, 3
 
 # struct Particle (src/Particle.h:4-10): 60 bytes.
 PARTICLE_DTYPE = np.dtype(
@@ -68,6 +70,32 @@ class SceneDesc(ctypes.Structure):
         ("cell_scale", c_float),
         ("flags", c_uint32),
     ]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("stencil_block", ctypes.c_int32), ("lane_walk", ctypes.c_int32), ("trials", ctypes.c_int32),
+                ("_pad", ctypes.c_int32), ("ms_per_step", c_double), ("default_ms_per_step", c_double),
+                ("trial_ms", c_double * 6)]
+
+    def as_dict(self):
+        return {"stencil_block": self.stencil_block, "lane_walk": self.lane_walk, "trials": self.trials,
+                "ms_per_step": self.ms_per_step, "default_ms_per_step": self.default_ms_per_step,
+                "trial_ms": list(self.trial_ms)}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("memory_budget", c_uint64), ("max_scenes", c_uint32), ("max_steps_per_tick", c_uint32),
+                ("flags", c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, c_uint64) for k in ("submitted", "admitted", "completed", "queued", "active", "steps",
+                                        "used_bytes", "budget_bytes")] + [
+        (k, c_double) for k in ("particle_steps", "scene_steps", "busy_ms", "wall_s", "step_ms_p50",
+                                "step_ms_p99", "step_ms_max")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SphError(RuntimeError):
@@ -123,6 +151,19 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_batch_scene_count": (c_uint32, [c_void_p]),
         "sph_batch_profile": (
             c_int, [c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_int), c_char_p, c_size_t]),
+        "sph_batch_step_timed": (c_int, [c_void_p, c_int, c_void_p]),
+        "sph_batch_autotune": (c_int, [c_void_p, c_int, POINTER(Tune)]),
+        "sph_hip_autotune": (c_int, [c_void_p, c_int, POINTER(Tune)]),
+        "sph_server_create": (c_int, [POINTER(ServerConfig), POINTER(c_void_p)]),
+        "sph_server_destroy": (c_int, [c_void_p]),
+        "sph_server_submit": (c_int, [c_void_p, POINTER(SceneDesc), c_void_p, c_void_p, c_uint32,
+                                      POINTER(c_uint64)]),
+        "sph_server_tick": (c_int, [c_void_p, POINTER(c_uint32)]),
+        "sph_server_status": (c_int, [c_void_p, c_uint64, POINTER(c_int), POINTER(c_uint32),
+                                      POINTER(c_double)]),
+        "sph_server_fetch": (c_int, [c_void_p, c_uint64] + [c_void_p] * 5),
+        "sph_server_stats": (c_int, [c_void_p, POINTER(ServerStats)]),
+        "sph_server_step_times": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -288,6 +329,11 @@ class Context:
         _check(self._lib.sph_hip_download_particles(self._h, _ptr(p), _ptr(xf)))
         return p, xf
 
+    def autotune(self, trial_steps: int = 10) -> dict:
+        t = Tune()
+        _check(self._lib.sph_hip_autotune(self._h, trial_steps, ctypes.byref(t)))
+        return t.as_dict()
+
     def profile(self, nsteps: int = 10) -> dict:
         ms = (c_double * 32)()
         nk = c_int(0)
@@ -363,6 +409,16 @@ class Batch:
                                             _ptr(ids)))
         return {"pos": pos, "vel": vel, "rho": rho, "pressure": prs, "ids": ids}
 
+    def step_timed(self, nsteps: int) -> np.ndarray:
+        ms = np.zeros(nsteps, np.float32)
+        _check(self._lib.sph_batch_step_timed(self._h, nsteps, _ptr(ms)))
+        return ms
+
+    def autotune(self, trial_steps: int = 10) -> dict:
+        t = Tune()
+        _check(self._lib.sph_batch_autotune(self._h, trial_steps, ctypes.byref(t)))
+        return t.as_dict()
+
     def profile(self, nsteps: int = 10) -> dict:
         ms = (c_double * 32)()
         nk = c_int(0)
@@ -370,6 +426,85 @@ class Batch:
         _check(self._lib.sph_batch_profile(self._h, nsteps, ms, 32, ctypes.byref(nk), names, 1024))
         labels = names.raw.split(b"\0")[: nk.value]
         return {labels[k].decode(): float(ms[k]) for k in range(nk.value)}
+
+
+class Server:
+    """Continuous-batching serving loop (sph_server_*): submit scene-advance
+    requests, call tick() to run scheduling rounds, fetch finished results."""
+
+    def __init__(self, memory_budget: int = 0, max_scenes: int = 0, max_steps_per_tick: int = 0,
+                 graph: bool = True):
+        self._lib = load()
+        cfg = ServerConfig(memory_budget, max_scenes, max_steps_per_tick, 0 if graph else SPH_SERVER_NO_GRAPH)
+        h = c_void_p()
+        _check(self._lib.sph_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.counts = {}
+
+    def close(self):
+        if self._h:
+            self._lib.sph_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, desc: SceneDesc, pos: np.ndarray, vel: np.ndarray | None, steps: int) -> int:
+        n = desc.n
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(n, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        rid = c_uint64(0)
+        _check(self._lib.sph_server_submit(self._h, ctypes.byref(desc), _ptr(pos), _ptr(vel), steps,
+                                           ctypes.byref(rid)))
+        self.counts[rid.value] = n
+        return rid.value
+
+    def tick(self) -> int:
+        ran = c_uint32(0)
+        _check(self._lib.sph_server_tick(self._h, ctypes.byref(ran)))
+        return ran.value
+
+    def status(self, rid: int):
+        st = c_int(0)
+        done = c_uint32(0)
+        t = (c_double * 3)()
+        _check(self._lib.sph_server_status(self._h, rid, ctypes.byref(st), ctypes.byref(done), t))
+        return st.value, done.value, (t[0], t[1], t[2])
+
+    def fetch(self, rid: int):
+        n = self.counts[rid]
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        rho = np.zeros(n, np.float32)
+        prs = np.zeros(n, np.float32)
+        ids = np.zeros(n, np.uint32)
+        _check(self._lib.sph_server_fetch(self._h, rid, _ptr(pos), _ptr(vel), _ptr(rho), _ptr(prs), _ptr(ids)))
+        return {"pos": pos, "vel": vel, "rho": rho, "pressure": prs, "ids": ids}
+
+    def discard(self, rid: int):
+        """Releases a finished request's result without copying it."""
+        _check(self._lib.sph_server_fetch(self._h, rid, None, None, None, None, None))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(self._lib.sph_server_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def step_times(self) -> np.ndarray:
+        n = c_size_t(0)
+        _check(self._lib.sph_server_step_times(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        _check(self._lib.sph_server_step_times(self._h, _ptr(out), n.value, ctypes.byref(n)))
+        return out
 
 
 def copy_bench(nbytes: int, iters: int = 20) -> float:

@@ -167,3 +167,58 @@ def blobs(nblobs: int = 12, per_blob: int = 2000, h: float = 0.05, sigma_h: floa
     mass = float(F32(0.02 * (h / 0.15) ** 3))
     return Scene(f"blobs-{nblobs}x{per_blob}", pos, vel, mass, 1000.0, 1.0, 1.04, h, -9.8, 0.2,
                  0.001, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), steps=200)
+
+
+def request_scene(n: int, kind: str, seed: int, dt: float = 0.001) -> Scene:
+    """One serving request's initial state in the box [-1,1]^3, h = 2.1
+    spacing (~32 interior neighbours), mass 0.02 (h / 0.15)^3:
+      dambreak  jittered lattice block (spacing 0.8 / ceil(cbrt n)) at the
+                low corner, half a spacing inside the reflection planes;
+      droplet   jittered lattice inside a sphere of radius 0.3 centred at
+                (0, 0.6, 0), spacing chosen so the sphere holds at least n
+                points, n of them kept at random."""
+    if kind == "dambreak":
+        side = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+        sp = 0.8 / side
+        h = 2.1 * sp
+        o = -1.0 + h + 0.5 * sp
+        pos = lattice((side, side, side), sp, [o, o, o], 0.01 * h, seed)[:n]
+    elif kind == "droplet":
+        r = 0.3
+        sp = (4.0 / 3.0 * math.pi * r ** 3 / n) ** (1.0 / 3.0)
+        while True:
+            m = int(math.ceil(2 * r / sp)) + 1
+            g = lattice((m, m, m), sp, [-0.5 * (m - 1) * sp] * 3, 0.0, seed)
+            inside = g[np.einsum("ij,ij->i", g.astype(np.float64), g.astype(np.float64)) <= r * r]
+            if len(inside) >= n:
+                break
+            sp *= 0.99
+        h = 2.1 * sp
+        rng = np.random.default_rng(seed)
+        keep = np.sort(rng.choice(len(inside), size=n, replace=False))  # whole sphere, lattice order
+        pos = (inside[keep].astype(np.float64) + rng.uniform(-0.01 * h, 0.01 * h, size=(n, 3))
+               + np.array([0.0, 0.6, 0.0])).astype(F32)
+    else:
+        raise ValueError(f"unknown request geometry {kind!r}")
+    mass = float(F32(0.02 * (h / 0.15) ** 3))
+    return Scene(f"{kind}-{n}", pos, np.zeros_like(pos), mass, 1000.0, 1.0, 1.04, h, -9.8, 0.2,
+                 dt, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+
+
+def request_stream(rate: float, duration: float, seed: int = 1024, n_lo: int = 2000, n_hi: int = 500000,
+                   steps_lo: int = 10, steps_hi: int = 1000):
+    """Poisson arrivals over [0, duration) at `rate` per second: a list of
+    (arrival seconds, particle count log-uniform in [n_lo, n_hi], steps
+    uniform in [steps_lo, steps_hi], geometry dambreak or droplet, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            break
+        n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        steps = int(rng.integers(steps_lo, steps_hi + 1))
+        kind = "dambreak" if rng.random() < 0.5 else "droplet"
+        out.append((t, n, steps, kind, int(rng.integers(0, 2**31 - 1))))
+    return out

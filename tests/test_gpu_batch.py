@@ -130,3 +130,20 @@ def test_batched_workload_runs(sph):
             d = b.download(k)
             assert np.array_equal(np.sort(d["ids"]), np.arange(sc.n))
             assert np.isfinite(d["pos"]).all()
+
+
+def test_autotune_restores_state_and_keeps_bits(sph):
+    scenes = sph_scenes.batched_workload(nscenes=6, seed=21) + [sph_scenes.blobs(nblobs=3, per_blob=1200)]
+    res = []
+    for tune in (True, False):
+        with sph.Batch([_desc(sph, sc) for sc in scenes]) as b:
+            for k, sc in enumerate(scenes):
+                b.upload(k, sc.pos, sc.vel)
+            if tune:
+                t = b.autotune(3)
+                assert t["stencil_block"] in (64, 128, 256) and t["lane_walk"] in (0, 1)
+                assert t["trials"] == 6 and min(t["trial_ms"]) == t["ms_per_step"] > 0
+            ms = b.step_timed(15)
+            assert ms.shape == (15,) and np.all(ms > 0)
+            res.append([_state(b.download(k)) for k in range(len(scenes))])
+    assert res[0] == res[1]
