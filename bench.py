@@ -125,6 +125,32 @@ def load_traffic(kernel):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def dropin_leg(sc, calls=5):
+    """The stateless drop-in (host particle array in, host array and render
+    transforms out, as updateParticles is called): PCIe-inclusive rate."""
+    lib = sph_amd.load()
+    p = sph_amd.make_particles(sc.pos, sc.vel)
+    xf = np.zeros((sc.n, 16), np.float32)
+    s = sph_amd.settings(*sc.settings_args())
+    b = sph_amd.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset)
+    import ctypes
+
+    def call():
+        rc = lib.sph_hip_update_particles_box(p.ctypes.data_as(ctypes.c_void_p), xf.ctypes.data_as(ctypes.c_void_p),
+                                              sc.n, ctypes.byref(s), sc.dt, ctypes.byref(b))
+        if rc != 0:
+            raise sph_amd.SphError(rc, lib.sph_hip_last_error().decode())
+
+    call()  # device buffers are allocated on the first call
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        call()
+    dt = (time.perf_counter() - t0) / calls
+    log(f"drop-in (host arrays, PCIe-inclusive): {dt * 1e3:.2f} ms/call, {sc.n / dt / 1e6:.1f} M particle-updates/s")
+    return {"ms_per_call": dt * 1e3, "particle_updates_per_s": sc.n / dt, "calls": calls,
+            "note": "stateless sph_hip_update_particles_box on host AoS + transforms, steps 1-6 of the job"}
+
+
 def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024):
     scenes = sph_scenes.batched_workload(nscenes, seed)
     descs = [sph_serving.scene_desc(sc) for sc in scenes]
@@ -262,6 +288,7 @@ def main():
         cpu = cpu_baseline(sc, job_states(sc, args.steps))
         log(f"cpu baseline: {cpu['value'] / 1e6:.3f} M particle-updates/s")
 
+    dropin = dropin_leg(sc)
     batched = None if args.no_batched else batched_leg(steps=args.batched_steps)
     serving = None
     if not args.no_serving:
@@ -289,6 +316,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "dropin_pcie_inclusive": dropin,
         "batched": batched,
         "serving": serving,
     }
