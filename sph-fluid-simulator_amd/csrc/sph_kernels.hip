@@ -589,35 +589,89 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
     return 4ull * un <= 5ull * own_max;
 }
 
-// Walks the wave-uniform range [b, e) as three runs: candidates before the
-// wave's own slots [lo, lo + 64), the own slots (where a lane meets
-// itself and must skip it), and those after.  run(j0, j1, self) walks
-// [j0, j1); self says whether the run can hold a lane's own slot.
-template <class Run>
-__device__ inline void split_self(uint32_t b, uint32_t e, uint32_t lo, Run run)
-{
-    const uint32_t hi = lo + 64u;
-    const uint32_t m0 = min(max(b, lo), e), m1 = min(max(b, hi), e);
-    if (b < m0)
-        run(b, m0, std::false_type{});
-    if (m0 < m1)
-        run(m0, m1, std::true_type{});
-    if (m1 < e)
-        run(m1, e, std::false_type{});
-}
+// Union walk staged through LDS.  The wave's union candidates, flattened
+// in row order (rows as for_each_row visits them, slots ascending), are
+// fetched 64 at a time -- one coalesced load per lane and record array, a
+// chunk ahead of their use -- into a per-wave LDS double buffer, then read
+// back by all lanes with one broadcast LDS read per record, so the
+// candidate loop never waits on a global load and never pays per-row
+// overhead.  Lanes past the end stage a sentinel (position +inf, slot ~0)
+// that fails every test, so chunks run in groups of four unchecked.
+template <int NV>
+struct StageBuf {
+    float4 rec[2][NV][64];
+    uint32_t slot[2][64];
+};
 
-// Calls row(b, e) for each row of the union box, in the order for_each_row
-// visits rows.  b and e are wave-uniform.
-template <class Row>
-__device__ inline void for_each_union_row(const uint32_t *__restrict__ cstart, const UnionRows &U,
-                                          const StepParams &P, Row row)
+template <int NV, class Visit>
+__device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, const UnionRows &U,
+                                         const StepParams &P, const float4 *const (&src)[NV],
+                                         StageBuf<NV> &sb, Visit visit)
 {
-    for (int z = U.z0; z <= U.z1; ++z)
-        for (int y = U.y0; y <= U.y1; ++y) {
-            const uint32_t r0 = row_base(P, y, z);
-            row(__builtin_amdgcn_readfirstlane(cstart[r0 + (uint32_t)U.x0]),
-                __builtin_amdgcn_readfirstlane(cstart[r0 + (uint32_t)U.x1 + 1u]));
+    const uint32_t lane = threadIdx.x & 63u;
+    const int ny = U.y1 - U.y0 + 1;
+    const int nrows = ny * (U.z1 - U.z0 + 1);  // at most 4 x 4
+    uint32_t b = 0, len = 0;
+    if ((int)lane < nrows) {
+        const uint32_t r0 = row_base(P, U.y0 + (int)lane % ny, U.z0 + (int)lane / ny);
+        b = cstart[r0 + (uint32_t)U.x0];
+        len = cstart[r0 + (uint32_t)U.x1 + 1u] - b;
+    }
+    const uint32_t end = wave_inclusive_scan(len);  // flattened end of the lane's row
+    const uint32_t total = __builtin_amdgcn_readlane(end, 63);
+    const uint32_t shift = b - (end - len);         // slot = flattened position + shift
+    auto fetch = [&](uint32_t f0, float4(&r)[NV], uint32_t &slot) {
+        const uint32_t f = f0 + lane;
+        uint32_t row = 0;
+        for (int k = 0; k < nrows; ++k)
+            row += f >= (uint32_t)__builtin_amdgcn_readlane(end, k) ? 1u : 0u;
+        const uint32_t sh = (uint32_t)__shfl((int)shift, (int)min(row, 63u), 64);
+        if (f < total) {
+            slot = f + sh;
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                r[v] = src[v][slot];
+        } else {
+            slot = 0xFFFFFFFFu;
+            r[0] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+#pragma unroll
+            for (int v = 1; v < NV; ++v)
+                r[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
+    };
+    const uint32_t nch = (total + 63u) >> 6;
+    float4 nr[NV];
+    uint32_t ns = 0;
+    if (nch)
+        fetch(0u, nr, ns);
+    for (uint32_t c = 0; c < nch; ++c) {
+        const uint32_t bs = c & 1u;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            sb.rec[bs][v][lane] = nr[v];
+        sb.slot[bs][lane] = ns;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (c + 1u < nch)
+            fetch((c + 1u) << 6, nr, ns);
+        const uint32_t cnt = min(64u, total - (c << 6));
+        const uint32_t cnt4 = (cnt + 3u) & ~3u;
+        for (uint32_t k = 0; k < cnt4; k += 4u) {
+            // all four records and slots in registers before the first use
+            float4 q[4][NV];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    q[i][v] = sb.rec[bs][v][k + i];
+            const uint4 j4 = *reinterpret_cast<const uint4 *>(&sb.slot[bs][k]);
+            visit(j4.x, q[0]);
+            visit(j4.y, q[1]);
+            visit(j4.z, q[2]);
+            visit(j4.w, q[3]);
+        }
+    }
 }
 
 // --------------------------------------------------------------- density
@@ -631,6 +685,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(
     float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, SceneSet S,
     NeighborDump dump)
 {
+    __shared__ StageBuf<1> stage[BLOCK / 64];
     Lane L;
     bool active;
     if (!wave_lanes<BLOCK>(S, L, active))
@@ -665,40 +720,31 @@ __global__ __launch_bounds__(BLOCK) void k_density(
         };
         UnionRows U;
         if (MODE == 0 && plan_union(cstart, cc, P, U)) {
-            // Wave-uniform candidate j: pos[j] is one scalar load for the wave.
+            // Wave-uniform candidate j, staged through LDS for the wave.
             const bool any_slow = __any(slow);
+            const float4 *const src[1] = {pos};
             auto walk = [&](auto slow_tag) {
                 constexpr bool AS = decltype(slow_tag)::value;
-                auto cand = [&](uint32_t j, float4 q, auto self_tag) {
+                staged_union_walk<1>(cstart, U, P, src, stage[threadIdx.x >> 6],
+                                     [&](uint32_t j, const float4(&r)[1]) {
+                    const float4 q = r[0];
                     float dx, dy, dz;
                     const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                    if (d2 < h2 && (!decltype(self_tag)::value || j != s)) {
-                        const float t = h2 - d2;
-                        const float term = mp6 * ((t * t) * t);
-                        if (AS) {
+                    const bool pass = d2 < h2 && j != s;
+                    const float t = h2 - d2;
+                    const float term = mp6 * ((t * t) * t);
+                    if (AS) {
+                        if (pass) {
                             const int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z,
                                                                      __float_as_uint(q.w) & 0xFFFFu)
                                                : 1;
                             acc += (float)m * term;
-                        } else {
-                            acc += term;  // (float)1 * term == term
                         }
+                    } else {
+                        const float a = acc + term;  // (float)1 * term == term
+                        acc = pass ? a : acc;
                     }
-                };
-                auto run = [&](uint32_t j0, uint32_t j1, auto self_tag) {
-                    const float4 *p = pos + j0;
-                    uint32_t j = j0;
-                    for (; j + 4u <= j1; j += 4u, p += 4) {
-                        const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
-                        cand(j, q0, self_tag);
-                        cand(j + 1u, q1, self_tag);
-                        cand(j + 2u, q2, self_tag);
-                        cand(j + 3u, q3, self_tag);
-                    }
-                    for (; j < j1; ++j, ++p)
-                        cand(j, p[0], self_tag);
-                };
-                for_each_union_row(cstart, U, P, [&](uint32_t b, uint32_t e) { split_self(b, e, L.first, run); });
+                });
             };
             if (any_slow)
                 walk(std::integral_constant<bool, true>{});
@@ -831,8 +877,8 @@ __device__ inline void force_pair(float d2, float dx, float dy, float dz, float 
 }
 
 // Two walks, chosen per wave by plan_union, with identical per-lane sums:
-//  * union: one wave-uniform candidate at a time (scalar loads), the pair
-//    term predicated on the lane's own test;
+//  * union: one wave-uniform candidate at a time (staged through LDS), the
+//    pair term predicated on the lane's own test;
 //  * per lane: candidates filtered kForceChunk at a time into a per-lane
 //    bitmask, the terms then run only over the set bits, so a wave pays for
 //    the most neighbours any of its lanes has in the chunk.
@@ -845,6 +891,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_integrate(
     uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA, uint32_t *__restrict__ counts,
     float4 *__restrict__ force_out, SceneSet S)
 {
+    __shared__ StageBuf<2> stage[BLOCK / 64];
     Lane L;
     bool active;
     if (!wave_lanes<BLOCK>(S, L, active))
@@ -866,35 +913,22 @@ __global__ __launch_bounds__(BLOCK) void k_force_integrate(
         UnionRows U;
         if (plan_union(cstart, cc, P, U)) {
             const bool any_slow = __any(slow);
+            const float4 *const src[2] = {posR, velR};
             auto walk = [&](auto slow_tag) {
                 constexpr bool AS = decltype(slow_tag)::value;
-                auto cand = [&](uint32_t j, float4 q, float4 w, auto self_tag) {
+                staged_union_walk<2>(cstart, U, P, src, stage[threadIdx.x >> 6],
+                                     [&](uint32_t j, const float4(&r)[2]) {
+                    const float4 q = r[0], w = r[1];  // (x_j, 0.5 / rho_j), (v_j, p_j)
                     float dx, dy, dz;
                     const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                    if (d2 < h2 && (!decltype(self_tag)::value || j != s)) {
+                    if (d2 < h2 && j != s) {
                         float am = q.w;  // q.w * 1.0f
                         if (AS && slow)
                             am = q.w * (float)bucket_multiplicity(ci.x, ci.y, ci.z,
                                                                   __float_as_uint(posB[j].w) & 0xFFFFu);
                         force_pair(d2, dx, dy, dz, am, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);
                     }
-                };
-                auto run = [&](uint32_t j0, uint32_t j1, auto self_tag) {
-                    const float4 *pq = posR + j0;  // wave-uniform: scalar loads
-                    const float4 *pw = velR + j0;
-                    uint32_t j = j0;
-                    for (; j + 4u <= j1; j += 4u, pq += 4, pw += 4) {
-                        const float4 q0 = pq[0], q1 = pq[1], q2 = pq[2], q3 = pq[3];
-                        const float4 w0 = pw[0], w1 = pw[1], w2 = pw[2], w3 = pw[3];
-                        cand(j, q0, w0, self_tag);
-                        cand(j + 1u, q1, w1, self_tag);
-                        cand(j + 2u, q2, w2, self_tag);
-                        cand(j + 3u, q3, w3, self_tag);
-                    }
-                    for (; j < j1; ++j, ++pq, ++pw)
-                        cand(j, pq[0], pw[0], self_tag);
-                };
-                for_each_union_row(cstart, U, P, [&](uint32_t b, uint32_t e) { split_self(b, e, L.first, run); });
+                });
             };
             if (any_slow)
                 walk(std::integral_constant<bool, true>{});
