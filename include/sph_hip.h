@@ -87,8 +87,10 @@ typedef struct sph_scene_desc_v1 {
 #define SPH_SCENE_KEEP_FORCE 1u
 
 /* Every wave of the density and force passes walks its candidates per lane
- * instead of choosing the wave-uniform walk where it pays.  The results are
- * bit-identical either way; the flag exists for tests and tuning. */
+ * instead of choosing the wave-uniform walk where it pays.  A wave task
+ * whose walk is not split into parts gives bit-identical results either
+ * way; a split one adds the same terms in another grouping (rounding-level
+ * differences).  The flag exists for tests. */
 #define SPH_SCENE_LANE_WALK 2u
 
 /* ---- host-only helpers (no device work) --------------------------------- */
@@ -196,20 +198,18 @@ int sph_batch_step_timed(sph_batch *b, int nsteps, float *ms);
 
 /* ---- autotuner --------------------------------------------------------------
  * Times trial_steps steps from the current state for each stencil block
- * size (64, 128, 256 threads) and walk mode (wave-uniform walk allowed, or
- * per-lane only), restoring the state before every trial and after the
- * last, and keeps the fastest for the following steps.  Neither knob
- * changes any result bit.  trial_ms[k] is the mean step time of trial k,
- * k = 3 * lane_walk + (0, 1, 2 for 64, 128, 256 threads).  The reference
- * has no counterpart (its thread count is hardware_concurrency,
- * src/sph.cpp:199). */
+ * size (64, 128, 256 threads), restoring the state before every trial and
+ * after the last, and keeps the fastest for the following steps.  The knob
+ * changes no result bit.  trial_ms[k] is the mean step time of trial k
+ * (k = 0, 1, 2 for 64, 128, 256 threads; 3-5 unused).  The reference has no
+ * counterpart (its thread count is hardware_concurrency, src/sph.cpp:199). */
 typedef struct sph_tune_v1 {
     int32_t stencil_block;       /* chosen threads per block */
-    int32_t lane_walk;           /* 1: per-lane walk chosen */
+    int32_t lane_walk;           /* always 0 (reserved) */
     int32_t trials;
     int32_t _pad;
     double ms_per_step;          /* mean step time of the chosen setting */
-    double default_ms_per_step;  /* same for 128 threads, wave-uniform walk */
+    double default_ms_per_step;  /* same for 128 threads */
     double trial_ms[6];
 } sph_tune_v1;
 

@@ -58,9 +58,8 @@ struct Batch {
     // the GPU time of step k.  Synchronises.
     int step_timed(int nsteps, float *ms);
     // Times trial_steps steps from the current state for every stencil
-    // block size (64, 128, 256) and walk mode (wave-uniform walk allowed or
-    // per-lane only), restoring the state before each trial and after the
-    // last, and keeps the fastest.  Results do not depend on either knob.
+    // block size (64, 128, 256), restoring the state before each trial and
+    // after the last, and keeps the fastest.  Results do not depend on it.
     int autotune(int trial_steps, sph_tune_v1 *out);
     int sync();
     int download(uint32_t s, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids);

@@ -32,13 +32,16 @@ namespace sph {
 
 constexpr int kBlock = 256;           // threads per block, particle kernels
 constexpr int kStencilBlock = 128;    // default threads per block, density / force
-constexpr int kForceChunk = 16;       // candidates filtered per bitmask in the force pass
+constexpr int kForceChunk = 8;        // candidates filtered per bitmask in the force pass
 constexpr int kScanItems = 16;        // cells per thread in the scan
 constexpr uint32_t kScanChunk = kBlock * kScanItems;  // cells per scan block
 constexpr uint32_t kSmallCell = 64;        // cells up to this size: stable rank by counting
 constexpr uint32_t kBigCellSortMax = 16384;  // larger cells keep arrival order
 constexpr int kBigSortBlock = 1024;        // threads of the big-cell sort
 constexpr int kBigSortBlocks = 256;        // blocks walking the big-cell list
+constexpr uint32_t kMaxParts = 16;         // parts of one split wave task, at most
+constexpr uint32_t kPartCandidates = 1024; // candidates per part, at least
+constexpr uint32_t kPartTasksPerWave = 4;  // part-task capacity per wave task
 
 struct ParticleArrays {
     float4 *posA, *velA, *posB, *velB;
@@ -68,6 +71,36 @@ struct SceneSet {
     uint32_t nwaves;
 };
 
+// Split wave tasks.  A wave task whose candidate walk is long (the union's
+// candidates, or the largest own stencil of a per-lane walk) runs as P
+// parts, each an equal slice of every lane's candidate sequence, with
+// P = ceil(length / part size) up to kMaxParts; the part size is
+// kPartCandidates, raised when the scene's total walk would need more than
+// kPartTasksPerWave part tasks per wave task (a scene's split depends on
+// the scene alone, so it is the same in any batch).  Parts are separate tasks pulled from a queue; the
+// last one to finish adds the partial sums in part order, so results are
+// deterministic.  An unsplit task sums in candidate order (and its two
+// walks agree bit for bit); a split one differs from it by rounding only.
+struct PartPlan {
+    uint32_t *nparts;    // per wave task, padded to a kScanChunk multiple (zero outside use)
+    uint32_t *off;       // exclusive scan of nparts; off[nwaves] = number of part tasks
+    uint32_t *scan_partials;
+    uint32_t padded;     // padded length of nparts
+    uint32_t *measure;   // walk length per wave task
+    unsigned long long *total;  // sum of the walk lengths per scene (zero between steps)
+    uint32_t nscenes;
+    uint32_t cap;        // part-task capacity (kPartTasksPerWave per wave task)
+    uint32_t *tasks;     // (wave task << 4) | part, cap entries
+    uint32_t *queue;     // pulled-task counters: [0] density, [1] force
+    float4 *buf;         // partial sums: (part task) * 64 + lane, cap * 64 entries
+    uint32_t *done;      // parts finished per wave task (zero between passes)
+};
+
+// Plans the parts of every wave task of S from the B-order cells and
+// cell ranges, and clears the task queues.
+void launch_plan_parts(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
+                       const PartPlan &pp);
+
 // Debug outputs of the density pass: per-particle neighbour count (with
 // multiplicity) and, when out != nullptr, the neighbour ids of row s at
 // out[off[s] ...].
@@ -91,10 +124,13 @@ void launch_sort_big_cells(hipStream_t st, const ParticleArrays &a, const CellAr
 void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S);
 // `block`: threads per block of the stencil passes, 64, 128 or 256 (the
 // dump modes always use kStencilBlock); results do not depend on it.
+// `parts` (may be null: every wave task unsplit, one wave each) is the
+// plan launch_plan_parts made for this step.
 void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                    const NeighborDump *dump, int block = kStencilBlock);
+                    const NeighborDump *dump, const PartPlan *parts = nullptr, int block = kStencilBlock);
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                            const SceneSet &S, float4 *force_out, bool next_keys, int block = kStencilBlock);
+                            const SceneSet &S, float4 *force_out, bool next_keys,
+                            const PartPlan *parts = nullptr, int block = kStencilBlock);
 // Drop-in output (single scene): order slots by the pre-step 16-bit bucket
 // (ties by id), and write the reference AoS layout plus transforms.
 void launch_h16_rank(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n);

@@ -23,6 +23,7 @@
 // contract (tolerance-checked sums).
 #include "sph_kernels.h"
 
+#include <algorithm>
 #include <type_traits>
 
 #pragma clang fp contract(off)
@@ -47,6 +48,7 @@ struct Lane {
     const StepParams *P;
     uint32_t slot;
     uint32_t first;  // first slot of the wave (wave-uniform)
+    uint32_t task;   // index of the wave task (wave-uniform)
 };
 
 template <int BLOCK>
@@ -68,24 +70,45 @@ __device__ inline bool lane_of(const SceneSet &S, Lane &out)
 // scene's last slot mirror the wave's first slot (always a valid slot) and
 // report active = false, so wave-wide votes and shuffles see defined data.
 // Returns false only for waves past the task list (the whole wave).
+// Wave task w (wave-uniform) with every lane live (see wave_lanes).
+__device__ inline void wave_task(const SceneSet &S, uint32_t w, Lane &out, bool &active)
+{
+    const WaveTask t = S.tasks[w];
+    const uint32_t scene = __builtin_amdgcn_readfirstlane(t.scene);
+    const uint32_t first = __builtin_amdgcn_readfirstlane(t.first);
+    out.P = S.params + scene;
+    out.first = first;
+    out.task = w;
+    const uint32_t slot = first + (threadIdx.x & 63u);
+    active = slot < out.P->p_end;
+    out.slot = active ? slot : first;
+}
+
 template <int BLOCK>
 __device__ inline bool wave_lanes(const SceneSet &S, Lane &out, bool &active)
 {
     const uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
     if (w >= S.nwaves)
         return false;
-    const WaveTask t = S.tasks[w];
-    const uint32_t scene = __builtin_amdgcn_readfirstlane(t.scene);
-    const uint32_t first = __builtin_amdgcn_readfirstlane(t.first);
-    out.P = S.params + scene;
-    out.first = first;
-    const uint32_t slot = first + (threadIdx.x & 63u);
-    active = slot < out.P->p_end;
-    out.slot = active ? slot : first;
+    wave_task(S, w, out, active);
     return true;
 }
 
 inline uint32_t blocks_for_waves(uint32_t nwaves, int block) { return (nwaves + block / 64 - 1) / (block / 64); }
+
+// Waves the device holds at once: CUs x 4 SIMDs x 8 wave slots.
+uint32_t resident_waves()
+{
+    static uint32_t cached = 0;
+    if (!cached) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached = 32u * (uint32_t)cus;
+    }
+    return cached;
+}
 
 // ---------------------------------------------------------------- utilities
 
@@ -505,6 +528,22 @@ __device__ inline void for_each_row(const uint32_t *__restrict__ cstart, uint32_
     }
 }
 
+// for_each_row restricted to the positions [lo, hi) of the lane's own
+// candidate sequence (its rows in order, slots ascending).
+template <class Row>
+__device__ inline void for_each_row_span(const uint32_t *__restrict__ cstart, uint32_t cc, const StepParams &P,
+                                         uint32_t lo, uint32_t hi, Row row)
+{
+    uint32_t off = 0;
+    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+        const uint32_t len = e - b;
+        const uint32_t s0 = max(lo, off), s1 = min(hi, off + len);
+        if (s0 < s1)
+            row(b + (s0 - off), b + (s1 - off));
+        off += len;
+    });
+}
+
 __device__ inline int wave_min_i(int x)
 {
 #pragma unroll
@@ -546,7 +585,7 @@ __device__ inline uint32_t row_base(const StepParams &P, int y, int z)
 // passing candidates in the same order (rows z-major, then y, slots
 // ascending) and give bit-identical sums.
 __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t cc, const StepParams &P,
-                                  UnionRows &U)
+                                  UnionRows &U, uint32_t &count)
 {
     if (P.lane_walk)
         return false;
@@ -575,18 +614,19 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
     U.y1 = y1 + 1 < P.ny ? y1 + 1 : P.ny - 1;
     U.z0 = z0 > 0 ? z0 - 1 : 0;
     U.z1 = z1 + 1 < P.nz ? z1 + 1 : P.nz - 1;
-    if (one)
-        return true;
     uint32_t un = 0;
     for (int z = U.z0; z <= U.z1; ++z)
         for (int y = U.y0; y <= U.y1; ++y) {
             const uint32_t r0 = row_base(P, y, z);
             un += cstart[r0 + (uint32_t)U.x1 + 1u] - cstart[r0 + (uint32_t)U.x0];
         }
+    count = __builtin_amdgcn_readfirstlane(un);
+    if (one)
+        return true;
     uint32_t own = 0;
     for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
     const uint32_t own_max = (uint32_t)wave_max_i((int)own);
-    return 4ull * un <= 5ull * own_max;
+    return 4ull * count <= 5ull * own_max;
 }
 
 // Union walk staged through LDS.  The wave's union candidates, flattened
@@ -603,10 +643,11 @@ struct StageBuf {
     uint32_t slot[2][64];
 };
 
+// Walks the positions [lo, hi) of the flattened sequence.
 template <int NV, class Visit>
 __device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, const UnionRows &U,
-                                         const StepParams &P, const float4 *const (&src)[NV],
-                                         StageBuf<NV> &sb, Visit visit)
+                                         const StepParams &P, uint32_t lo, uint32_t hi,
+                                         const float4 *const (&src)[NV], StageBuf<NV> &sb, Visit visit)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const int ny = U.y1 - U.y0 + 1;
@@ -618,7 +659,7 @@ __device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, co
         len = cstart[r0 + (uint32_t)U.x1 + 1u] - b;
     }
     const uint32_t end = wave_inclusive_scan(len);  // flattened end of the lane's row
-    const uint32_t total = __builtin_amdgcn_readlane(end, 63);
+    const uint32_t total = min(__builtin_amdgcn_readlane(end, 63), hi);
     const uint32_t shift = b - (end - len);         // slot = flattened position + shift
     auto fetch = [&](uint32_t f0, float4(&r)[NV], uint32_t &slot) {
         const uint32_t f = f0 + lane;
@@ -639,11 +680,11 @@ __device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, co
                 r[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     };
-    const uint32_t nch = (total + 63u) >> 6;
+    const uint32_t nch = total > lo ? (total - lo + 63u) >> 6 : 0u;
     float4 nr[NV];
     uint32_t ns = 0;
     if (nch)
-        fetch(0u, nr, ns);
+        fetch(lo, nr, ns);
     for (uint32_t c = 0; c < nch; ++c) {
         const uint32_t bs = c & 1u;
 #pragma unroll
@@ -654,8 +695,8 @@ __device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, co
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (c + 1u < nch)
-            fetch((c + 1u) << 6, nr, ns);
-        const uint32_t cnt = min(64u, total - (c << 6));
+            fetch(lo + ((c + 1u) << 6), nr, ns);
+        const uint32_t cnt = min(64u, total - lo - (c << 6));
         const uint32_t cnt4 = (cnt + 3u) & ~3u;
         for (uint32_t k = 0; k < cnt4; k += 4u) {
             // all four records and slots in registers before the first use
@@ -674,23 +715,200 @@ __device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, co
     }
 }
 
+// ------------------------------------------------------------ part plan
+
+// Walk length of every wave task (the same decision the stencil passes
+// make: union count, or the largest own stencil), and their sum per scene:
+// the block's waves add theirs in LDS first, so runs of one scene cost one
+// global atomic per block.
+constexpr int kPlanBlock = 1024;
+
+__global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__restrict__ cellcB,
+                                                           const uint32_t *__restrict__ cstart, SceneSet S,
+                                                           uint32_t *__restrict__ measure,
+                                                           unsigned long long *__restrict__ total)
+{
+    __shared__ uint32_t s_scene[kPlanBlock / 64], s_len[kPlanBlock / 64];
+    const uint32_t wave = threadIdx.x >> 6;
+    Lane L;
+    bool active;
+    if (wave_lanes<kPlanBlock>(S, L, active)) {
+        const StepParams &P = *L.P;
+        const uint32_t cc = cellcB[L.slot];
+        UnionRows U;
+        uint32_t ucount = 0, len;
+        if (plan_union(cstart, cc, P, U, ucount) && !__any((cc & kSlowBit) != 0u)) {
+            len = ucount;
+        } else {
+            uint32_t own = 0;
+            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+            len = (uint32_t)wave_max_i((int)own);
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            measure[L.task] = len;
+            s_scene[wave] = S.tasks[L.task].scene;
+            s_len[wave] = len;
+        }
+    } else if ((threadIdx.x & 63u) == 0u) {
+        s_scene[wave] = 0xFFFFFFFFu;
+        s_len[wave] = 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t scene = s_scene[0];
+        unsigned long long sum = 0;
+        for (int k = 0; k < kPlanBlock / 64; ++k) {
+            if (s_scene[k] != scene) {
+                if (scene != 0xFFFFFFFFu)
+                    atomicAdd(&total[scene], sum);
+                scene = s_scene[k];
+                sum = 0;
+            }
+            sum += s_len[k];
+        }
+        if (scene != 0xFFFFFFFFu)
+            atomicAdd(&total[scene], sum);
+    }
+}
+
+// Parts per wave task: the part size grows beyond kPartCandidates when the
+// scene's total walk would need more than kPartTasksPerWave part tasks per
+// wave task of the scene.
+__global__ void k_count_parts(const uint32_t *__restrict__ measure, SceneSet S,
+                              const unsigned long long *__restrict__ total, uint32_t *__restrict__ nparts)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.nwaves)
+        return;
+    const uint32_t scene = S.tasks[g].scene;
+    const StepParams &P = S.params[scene];
+    const unsigned long long waves = (P.p_end - P.p_begin + 63u) / 64u;
+    const unsigned long long room = (kPartTasksPerWave - 1u) * waves;
+    const unsigned long long size = max((unsigned long long)kPartCandidates, (total[scene] + room - 1u) / room);
+    const unsigned long long p = (measure[g] + size - 1u) / size;
+    nparts[g] = (uint32_t)min(max(p, 1ull), (unsigned long long)kMaxParts);
+}
+
+__global__ void k_emit_parts(const uint32_t *__restrict__ off, uint32_t ntasks, uint32_t *__restrict__ tasks,
+                             uint32_t *__restrict__ queue, unsigned long long *__restrict__ total,
+                             uint32_t nscenes)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) {
+        queue[0] = 0u;
+        queue[1] = 0u;
+    }
+    for (uint32_t k = g; k < nscenes; k += gridDim.x * blockDim.x)
+        total[k] = 0ull;  // for the next step's plan
+    if (g >= ntasks)
+        return;
+    const uint32_t b = off[g], e = off[g + 1];
+    for (uint32_t p = 0; b + p < e; ++p)
+        tasks[b + p] = (g << 4) | p;
+}
+
+void launch_plan_parts(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
+                       const PartPlan &pp)
+{
+    hipLaunchKernelGGL(k_plan_parts, dim3(blocks_for_waves(S.nwaves, kPlanBlock)), dim3(kPlanBlock), 0, st,
+                       a.cellcB, c.cstart, S, pp.measure, pp.total);
+    hipLaunchKernelGGL(k_count_parts, dim3(blocks_for(S.nwaves)), dim3(kBlock), 0, st, pp.measure, S, pp.total,
+                       pp.nparts);
+    CellArrays cs;
+    cs.counts = pp.nparts;
+    cs.cstart = pp.off;
+    cs.partials = pp.scan_partials;
+    cs.big = nullptr;
+    cs.big_cap = 0;
+    cs.ncells_padded = pp.padded;
+    launch_scan(st, cs);  // also clears nparts for the next step
+    hipLaunchKernelGGL(k_emit_parts, dim3(blocks_for(S.nwaves)), dim3(kBlock), 0, st, pp.off, S.nwaves, pp.tasks,
+                       pp.queue, pp.total, pp.nscenes);
+}
+
+// Runs task(w, part, nparts) for the work the calling wave takes.  Without
+// a plan each wave runs wave task w of its own index, unsplit.  With one,
+// waves pull part tasks from the queue counter until the list is exhausted
+// (every wave exits then).  All 64 lanes must be live on entry.
+template <int BLOCK, class Task>
+__device__ inline void for_each_part(const SceneSet &S, const PartPlan &pp, uint32_t *queue, Task task)
+{
+    if (!queue) {
+        const uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
+        if (w < S.nwaves)
+            task(w, 0u, 1u);
+        return;
+    }
+    const uint32_t total = __builtin_amdgcn_readfirstlane(pp.off[S.nwaves]);
+    for (;;) {
+        uint32_t t = 0;
+        if ((threadIdx.x & 63u) == 0u)
+            t = atomicAdd(queue, 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= total)
+            return;
+        const uint32_t packed = __builtin_amdgcn_readfirstlane(pp.tasks[t]);
+        const uint32_t w = packed >> 4, part = packed & 15u;
+        const uint32_t nparts = __builtin_amdgcn_readfirstlane(pp.off[w + 1] - pp.off[w]);
+        task(w, part, nparts);
+    }
+}
+
+__device__ inline float load_agent(const float *p)
+{
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t *>(p), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Part `part` of wave task w stores its N partial sums; the last part to
+// finish gets every part's sums added in part order in v and returns true
+// (the others return false).  Every lane of the wave calls it.
+template <int N>
+__device__ inline bool combine_parts(const PartPlan &pp, uint32_t w, uint32_t part, uint32_t nparts, float (&v)[N])
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t base = __builtin_amdgcn_readfirstlane(pp.off[w]);
+    float *mine = reinterpret_cast<float *>(pp.buf + ((size_t)base + part) * 64u + lane);
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        mine[k] = v[k];
+    __threadfence();
+    uint32_t prev = 0;
+    if (lane == 0u)
+        prev = atomicAdd(&pp.done[w], 1u);
+    prev = __builtin_amdgcn_readfirstlane(prev);
+    if (prev + 1u != nparts)
+        return false;
+    __threadfence();
+    for (uint32_t q = 0; q < nparts; ++q) {
+        const float *src = reinterpret_cast<const float *>(pp.buf + ((size_t)base + q) * 64u + lane);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const float x = load_agent(src + k);
+            v[k] = q == 0u ? x : v[k] + x;
+        }
+    }
+    if (lane == 0u)
+        pp.done[w] = 0u;  // ready for the next pass
+    return true;
+}
+
 // --------------------------------------------------------------- density
 // MODE 0: density + pressure.  MODE 1: also the neighbour count per
 // particle.  MODE 2: also the neighbour ids (multiplicity-repeated).
 // Writes rho, posR = (x, 0.5 / rho) and velR = (v, p) for the force pass.
 template <int MODE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_density(
+__global__ __launch_bounds__(BLOCK, 6) void k_density(
     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
     const uint32_t *__restrict__ cellc, const uint32_t *__restrict__ cstart,
     float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, SceneSet S,
-    NeighborDump dump)
+    NeighborDump dump, PartPlan pp)
 {
     __shared__ StageBuf<1> stage[BLOCK / 64];
-    Lane L;
-    bool active;
-    if (!wave_lanes<BLOCK>(S, L, active))
-        return;
-    {
+    for_each_part<BLOCK>(S, pp, pp.queue, [&](uint32_t wt, uint32_t part, uint32_t nparts) {
+        Lane L;
+        bool active;
+        wave_task(S, wt, L, active);
         const StepParams &P = *L.P;
         const uint32_t s = L.slot;
         const float4 pi = pos[s];
@@ -705,7 +923,12 @@ __global__ __launch_bounds__(BLOCK) void k_density(
             float dx, dy, dz;
             float d2 = dist2_ref(pi, q, dx, dy, dz);
             if (d2 < h2) {
-                int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z, __float_as_uint(q.w) & 0xFFFFu) : 1;
+                int m = 1;
+                if (slow) {
+                    int cx = ci.x, cy = ci.y, cz = ci.z;
+                    asm volatile("" : "+v"(cx), "+v"(cy), "+v"(cz));
+                    m = bucket_multiplicity(cx, cy, cz, __float_as_uint(q.w) & 0xFFFFu);
+                }
                 float t = h2 - d2;
                 float term = mp6 * ((t * t) * t);
                 acc += (float)m * term;
@@ -718,40 +941,34 @@ __global__ __launch_bounds__(BLOCK) void k_density(
                 }
             }
         };
+        // Waves with a slow-multiplicity lane (rare where the union pays)
+        // take the per-lane walk, so the union walk needs no m_ij.  A split
+        // task walks its slice [part, part + 1) / nparts of the sequence.
         UnionRows U;
-        if (MODE == 0 && plan_union(cstart, cc, P, U)) {
+        uint32_t ucount = 0;
+        if (MODE == 0 && plan_union(cstart, cc, P, U, ucount) && !__any(slow)) {
             // Wave-uniform candidate j, staged through LDS for the wave.
-            const bool any_slow = __any(slow);
             const float4 *const src[1] = {pos};
-            auto walk = [&](auto slow_tag) {
-                constexpr bool AS = decltype(slow_tag)::value;
-                staged_union_walk<1>(cstart, U, P, src, stage[threadIdx.x >> 6],
-                                     [&](uint32_t j, const float4(&r)[1]) {
-                    const float4 q = r[0];
-                    float dx, dy, dz;
-                    const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                    const bool pass = d2 < h2 && j != s;
-                    const float t = h2 - d2;
-                    const float term = mp6 * ((t * t) * t);
-                    if (AS) {
-                        if (pass) {
-                            const int m = slow ? bucket_multiplicity(ci.x, ci.y, ci.z,
-                                                                     __float_as_uint(q.w) & 0xFFFFu)
-                                               : 1;
-                            acc += (float)m * term;
-                        }
-                    } else {
-                        const float a = acc + term;  // (float)1 * term == term
-                        acc = pass ? a : acc;
-                    }
-                });
-            };
-            if (any_slow)
-                walk(std::integral_constant<bool, true>{});
-            else
-                walk(std::integral_constant<bool, false>{});
+            const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
+            const uint32_t hi = (uint32_t)((uint64_t)ucount * (part + 1u) / nparts);
+            staged_union_walk<1>(cstart, U, P, lo, hi, src, stage[threadIdx.x >> 6],
+                                 [&](uint32_t j, const float4(&r)[1]) {
+                const float4 q = r[0];
+                float dx, dy, dz;
+                const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                const float t = h2 - d2;
+                const float a = acc + mp6 * ((t * t) * t);  // (float)1 * term == term
+                acc = (d2 < h2 && j != s) ? a : acc;
+            });
         } else {
-            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+            uint32_t lo = 0, hi = 0xFFFFFFFFu;
+            if (nparts > 1u) {
+                uint32_t own = 0;
+                for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+                lo = (uint32_t)((uint64_t)own * part / nparts);
+                hi = (uint32_t)((uint64_t)own * (part + 1u) / nparts);
+            }
+            for_each_row_span(cstart, cc, P, lo, hi, [&](uint32_t b, uint32_t e) {
                 uint32_t j = b;
                 for (; j + 1u < e; j += 2u) {  // two candidates in flight
                     const float4 q0 = pos[j];
@@ -765,6 +982,12 @@ __global__ __launch_bounds__(BLOCK) void k_density(
                     body(j, pos[j]);
             });
         }
+        if (nparts > 1u) {
+            float v[1] = {acc};
+            if (!combine_parts<1>(pp, wt, part, nparts, v))
+                return;
+            acc = v[0];
+        }
         if (!active)
             return;
         const float r = acc + P.self_dens;
@@ -775,31 +998,41 @@ __global__ __launch_bounds__(BLOCK) void k_density(
         velR[s] = make_float4(vs.x, vs.y, vs.z, p);
         if (MODE >= 1)
             dump.count[s] = cnt;
-    }
+    });
+}
+
+// Grid of a stencil pass: one wave per wave task, or with a part plan as
+// many waves as the device holds at once (they pull the part tasks).
+inline uint32_t stencil_waves(const SceneSet &S, const PartPlan *pp)
+{
+    return pp ? std::min<uint32_t>(pp->cap, resident_waves()) : S.nwaves;
 }
 
 template <int MODE, int BLOCK>
 static void launch_density_block(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                                 const SceneSet &S, const NeighborDump &d)
+                                 const SceneSet &S, const NeighborDump &d, const PartPlan *pp)
 {
-    hipLaunchKernelGGL((k_density<MODE, BLOCK>), dim3(blocks_for_waves(S.nwaves, BLOCK)), dim3(BLOCK), 0, st,
-                       a.posB, a.velB, a.cellcB, c.cstart, a.rhoB, a.posR, a.velR, S, d);
+    PartPlan p{};
+    if (pp)
+        p = *pp;
+    hipLaunchKernelGGL((k_density<MODE, BLOCK>), dim3(std::max(1u, blocks_for_waves(stencil_waves(S, pp), BLOCK))),
+                       dim3(BLOCK), 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB, a.posR, a.velR, S, d, p);
 }
 
 void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                    const NeighborDump *dump, int block)
+                    const NeighborDump *dump, const PartPlan *parts, int block)
 {
     NeighborDump d = dump ? *dump : NeighborDump{nullptr, nullptr, nullptr};
-    if (dump)
-        return dump->out ? launch_density_block<2, kStencilBlock>(st, a, c, S, d)
-                         : launch_density_block<1, kStencilBlock>(st, a, c, S, d);
+    if (dump)  // the dump modes walk every lane's whole stencil in one task
+        return dump->out ? launch_density_block<2, kStencilBlock>(st, a, c, S, d, nullptr)
+                         : launch_density_block<1, kStencilBlock>(st, a, c, S, d, nullptr);
     switch (block) {
     case 64:
-        return launch_density_block<0, 64>(st, a, c, S, d);
+        return launch_density_block<0, 64>(st, a, c, S, d, parts);
     case 256:
-        return launch_density_block<0, 256>(st, a, c, S, d);
+        return launch_density_block<0, 256>(st, a, c, S, d, parts);
     default:
-        return launch_density_block<0, kStencilBlock>(st, a, c, S, d);
+        return launch_density_block<0, kStencilBlock>(st, a, c, S, d, parts);
     }
 }
 
@@ -883,20 +1116,20 @@ __device__ inline void force_pair(float d2, float dx, float dy, float dz, float 
 //    bitmask, the terms then run only over the set bits, so a wave pays for
 //    the most neighbours any of its lanes has in the chunk.
 template <bool WRITE_FORCE, bool NEXT_KEYS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_force_integrate(
+__global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
     const float4 *__restrict__ posB, const float4 *__restrict__ velB,
     const float4 *__restrict__ posR, const float4 *__restrict__ velR,
     const uint32_t *__restrict__ cellcB, const uint32_t *__restrict__ cstart,
     const float *__restrict__ rhoB, float4 *__restrict__ posA, float4 *__restrict__ velA,
     uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA, uint32_t *__restrict__ counts,
-    float4 *__restrict__ force_out, SceneSet S)
+    float4 *__restrict__ force_out, SceneSet S, PartPlan pp)
 {
     __shared__ StageBuf<2> stage[BLOCK / 64];
-    Lane L;
-    bool active;
-    if (!wave_lanes<BLOCK>(S, L, active))
-        return;
-    {
+    for_each_part<BLOCK>(S, pp, pp.queue ? pp.queue + 1 : nullptr, [&](uint32_t wt, uint32_t part,
+                                                                      uint32_t nparts) {
+        Lane L;
+        bool active;
+        wave_task(S, wt, L, active);
         const StepParams &P = *L.P;
         const uint32_t s = L.slot;
         float4 pi = posR[s];        // w = 0.5 / rho_i
@@ -911,31 +1144,28 @@ __global__ __launch_bounds__(BLOCK) void k_force_integrate(
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 
         UnionRows U;
-        if (plan_union(cstart, cc, P, U)) {
-            const bool any_slow = __any(slow);
+        uint32_t ucount = 0;
+        if (plan_union(cstart, cc, P, U, ucount) && !__any(slow)) {  // as in k_density
             const float4 *const src[2] = {posR, velR};
-            auto walk = [&](auto slow_tag) {
-                constexpr bool AS = decltype(slow_tag)::value;
-                staged_union_walk<2>(cstart, U, P, src, stage[threadIdx.x >> 6],
-                                     [&](uint32_t j, const float4(&r)[2]) {
-                    const float4 q = r[0], w = r[1];  // (x_j, 0.5 / rho_j), (v_j, p_j)
-                    float dx, dy, dz;
-                    const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                    if (d2 < h2 && j != s) {
-                        float am = q.w;  // q.w * 1.0f
-                        if (AS && slow)
-                            am = q.w * (float)bucket_multiplicity(ci.x, ci.y, ci.z,
-                                                                  __float_as_uint(posB[j].w) & 0xFFFFu);
-                        force_pair(d2, dx, dy, dz, am, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);
-                    }
-                });
-            };
-            if (any_slow)
-                walk(std::integral_constant<bool, true>{});
-            else
-                walk(std::integral_constant<bool, false>{});
+            const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
+            const uint32_t hi = (uint32_t)((uint64_t)ucount * (part + 1u) / nparts);
+            staged_union_walk<2>(cstart, U, P, lo, hi, src, stage[threadIdx.x >> 6],
+                                 [&](uint32_t j, const float4(&r)[2]) {
+                const float4 q = r[0], w = r[1];  // (x_j, 0.5 / rho_j), (v_j, p_j)
+                float dx, dy, dz;
+                const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                if (d2 < h2 && j != s)
+                    force_pair(d2, dx, dy, dz, q.w, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);  // am = q.w * 1
+            });
         } else {
-            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+            uint32_t lo = 0, hi = 0xFFFFFFFFu;
+            if (nparts > 1u) {
+                uint32_t own = 0;
+                for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+                lo = (uint32_t)((uint64_t)own * part / nparts);
+                hi = (uint32_t)((uint64_t)own * (part + 1u) / nparts);
+            }
+            for_each_row_span(cstart, cc, P, lo, hi, [&](uint32_t b, uint32_t e) {
                 for (uint32_t jj = b; jj < e; jj += kForceChunk) {
                     uint32_t mask = 0u;
 #pragma unroll
@@ -956,13 +1186,23 @@ __global__ __launch_bounds__(BLOCK) void k_force_integrate(
                         float dx, dy, dz;
                         const float d2 = dist2_ref(pi, q, dx, dy, dz);
                         float am = q.w;
-                        if (slow)
-                            am = q.w * (float)bucket_multiplicity(ci.x, ci.y, ci.z,
-                                                                  __float_as_uint(posB[j].w) & 0xFFFFu);
+                        if (slow) {
+                            int cx = ci.x, cy = ci.y, cz = ci.z;
+                            asm volatile("" : "+v"(cx), "+v"(cy), "+v"(cz));
+                            am = q.w * (float)bucket_multiplicity(cx, cy, cz, __float_as_uint(posB[j].w) & 0xFFFFu);
+                        }
                         force_pair(d2, dx, dy, dz, am, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);
                     }
                 }
             });
+        }
+        if (nparts > 1u) {
+            float v[3] = {fx, fy, fz};
+            if (!combine_parts<3>(pp, wt, part, nparts, v))
+                return;
+            fx = v[0];
+            fy = v[1];
+            fz = v[2];
         }
         if (!active)
             return;
@@ -979,18 +1219,21 @@ __global__ __launch_bounds__(BLOCK) void k_force_integrate(
         }
         posA[s] = pi;
         velA[s] = vi;
-    }
+    });
 }
 
 template <int BLOCK>
 static void launch_force_block(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                               float4 *force_out, bool next_keys)
+                               float4 *force_out, bool next_keys, const PartPlan *pp)
 {
-    dim3 g(blocks_for_waves(S.nwaves, BLOCK)), b(BLOCK);
+    PartPlan p{};
+    if (pp)
+        p = *pp;
+    dim3 g(std::max(1u, blocks_for_waves(stencil_waves(S, pp), BLOCK))), b(BLOCK);
 #define SPH_LAUNCH_FI(WF, NK)                                                                          \
     hipLaunchKernelGGL((k_force_integrate<WF, NK, BLOCK>), g, b, 0, st, a.posB, a.velB, a.posR, a.velR, \
                        a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,         \
-                       force_out, S)
+                       force_out, S, p)
     if (force_out && next_keys)
         SPH_LAUNCH_FI(true, true);
     else if (force_out)
@@ -1003,15 +1246,16 @@ static void launch_force_block(hipStream_t st, const ParticleArrays &a, const Ce
 }
 
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                            const SceneSet &S, float4 *force_out, bool next_keys, int block)
+                            const SceneSet &S, float4 *force_out, bool next_keys, const PartPlan *parts,
+                            int block)
 {
     switch (block) {
     case 64:
-        return launch_force_block<64>(st, a, c, S, force_out, next_keys);
+        return launch_force_block<64>(st, a, c, S, force_out, next_keys, parts);
     case 256:
-        return launch_force_block<256>(st, a, c, S, force_out, next_keys);
+        return launch_force_block<256>(st, a, c, S, force_out, next_keys, parts);
     default:
-        return launch_force_block<kStencilBlock>(st, a, c, S, force_out, next_keys);
+        return launch_force_block<kStencilBlock>(st, a, c, S, force_out, next_keys, parts);
     }
 }
 

@@ -43,9 +43,15 @@ struct Workspace {
     DevBuf counts, cstart, partials, big;
     DevBuf force;
     DevBuf params, tasks;  // scene table of the batch living in this workspace
+    // split wave tasks: parts per task (padded), their scan, scan partials,
+    // walk lengths and their total, part tasks, queue counters, partial sums,
+    // finished-part counters
+    DevBuf pnparts, poff, ppartials, pmeasure, ptotal, ptasks, pqueue, pbuf, pdone;
+    uint32_t parts_padded = 0;
     uint32_t ncells_padded = 0;
     uint32_t big_cap = 0;
     uint32_t nwaves = 0;
+    uint32_t nscenes = 0;
     int stencil_block = kStencilBlock;  // threads per block of the density and force passes
     int reserve_particles(uint32_t n);
     int reserve_cells(uint64_t ncells);
@@ -55,6 +61,7 @@ struct Workspace {
     ParticleArrays particles() const;
     CellArrays cells() const;
     SceneSet scene_set() const;
+    PartPlan part_plan() const;
     void release_all();
 };
 

@@ -262,6 +262,23 @@ int Workspace::set_scenes(const std::vector<StepParams> &scenes, hipStream_t st)
     if ((rc = params.reserve(sizeof(StepParams) * std::max<size_t>(scenes.size(), 1))) ||
         (rc = tasks.reserve(sizeof(WaveTask) * std::max<size_t>(t.size(), 1))))
         return rc;
+    const size_t nw = std::max<size_t>(t.size(), 1);
+    const uint32_t padded = (uint32_t)((nw + kScanChunk - 1) / kScanChunk * kScanChunk);
+    const size_t cap = kPartTasksPerWave * nw;
+    const bool fresh_counters = padded > parts_padded || pdone.bytes < 4 * nw ||
+                                ptotal.bytes < 8 * std::max<size_t>(scenes.size(), 1);
+    if ((rc = pnparts.reserve(4ull * padded)) || (rc = poff.reserve(4ull * (padded + 1))) ||
+        (rc = ppartials.reserve(4ull * (padded / kScanChunk))) || (rc = pmeasure.reserve(4ull * nw)) ||
+        (rc = ptotal.reserve(8ull * std::max<size_t>(scenes.size(), 1))) || (rc = ptasks.reserve(4ull * cap)) ||
+        (rc = pqueue.reserve(16)) ||
+        (rc = pbuf.reserve(sizeof(float4) * 64ull * cap)) || (rc = pdone.reserve(4ull * nw)))
+        return rc;
+    if (fresh_counters) {  // zero outside use from here on: the step keeps them so
+        launch_fill_u32(st, (uint32_t *)pnparts.ptr, 0u, pnparts.bytes / 4);
+        launch_fill_u32(st, (uint32_t *)pdone.ptr, 0u, pdone.bytes / 4);
+        launch_fill_u32(st, (uint32_t *)ptotal.ptr, 0u, ptotal.bytes / 4);
+        parts_padded = (uint32_t)(pnparts.bytes / 4 / kScanChunk * kScanChunk);
+    }
     hipError_t e = hipSuccess;
     if (!scenes.empty())
         e = hipMemcpyAsync(params.ptr, scenes.data(), sizeof(StepParams) * scenes.size(),
@@ -273,6 +290,7 @@ int Workspace::set_scenes(const std::vector<StepParams> &scenes, hipStream_t st)
     if (e != hipSuccess)
         return hip_fail(e, "scene table upload");
     nwaves = (uint32_t)t.size();
+    nscenes = (uint32_t)scenes.size();
     return SPH_OK;
 }
 
@@ -314,14 +332,35 @@ SceneSet Workspace::scene_set() const
     return S;
 }
 
+PartPlan Workspace::part_plan() const
+{
+    PartPlan p;
+    p.nparts = (uint32_t *)pnparts.ptr;
+    p.off = (uint32_t *)poff.ptr;
+    p.scan_partials = (uint32_t *)ppartials.ptr;
+    p.padded = (nwaves + kScanChunk - 1) / kScanChunk * kScanChunk;
+    p.measure = (uint32_t *)pmeasure.ptr;
+    p.total = (unsigned long long *)ptotal.ptr;
+    p.nscenes = nscenes;
+    p.cap = kPartTasksPerWave * std::max(nwaves, 1u);
+    p.tasks = (uint32_t *)ptasks.ptr;
+    p.queue = (uint32_t *)pqueue.ptr;
+    p.buf = (float4 *)pbuf.ptr;
+    p.done = (uint32_t *)pdone.ptr;
+    return p;
+}
+
 void Workspace::release_all()
 {
     for (DevBuf *b : {&posA, &velA, &posB, &velB, &cellcA, &rankA, &cellcB, &perm, &rhoB, &posR, &velR,
-                      &counts, &cstart, &partials, &big, &force, &params, &tasks})
+                      &counts, &cstart, &partials, &big, &force, &params, &tasks, &pnparts, &poff,
+                      &ppartials, &pmeasure, &ptotal, &ptasks, &pqueue, &pbuf, &pdone})
         b->release();
     ncells_padded = 0;
     big_cap = 0;
     nwaves = 0;
+    nscenes = 0;
+    parts_padded = 0;
 }
 
 void enqueue_step(hipStream_t st, Workspace &w, float4 *force_out, bool next_keys, StepTimer *timer)
@@ -339,12 +378,14 @@ void enqueue_step(hipStream_t st, Workspace &w, float4 *force_out, bool next_key
     if (timer)
         timer->mark(st, "scatter");
     launch_reorder(st, a, c, S);
+    const PartPlan pp = w.part_plan();
+    launch_plan_parts(st, a, c, S, pp);
     if (timer)
         timer->mark(st, "reorder");
-    launch_density(st, a, c, S, nullptr, w.stencil_block);
+    launch_density(st, a, c, S, nullptr, &pp, w.stencil_block);
     if (timer)
         timer->mark(st, "density");
-    launch_force_integrate(st, a, c, S, force_out, next_keys, w.stencil_block);
+    launch_force_integrate(st, a, c, S, force_out, next_keys, &pp, w.stencil_block);
     if (timer)
         timer->mark(st, "force_integrate");
 }
@@ -498,8 +539,10 @@ int update_particles(sph_particle_v1 *particles, float *transforms, size_t n_,
     ParticleArrays a = w.particles();
     CellArrays c = w.cells();
     const SceneSet set = w.scene_set();
-    launch_density(st, a, c, set, nullptr);
-    launch_force_integrate(st, a, c, set, (float4 *)w.force.ptr, /*next_keys=*/false);
+    const PartPlan pp = w.part_plan();
+    launch_plan_parts(st, a, c, set, pp);
+    launch_density(st, a, c, set, nullptr, &pp);
+    launch_force_integrate(st, a, c, set, (float4 *)w.force.ptr, /*next_keys=*/false, &pp);
     CellArrays c16 = cells16(S);
     launch_fill_u32(st, c16.counts, 0u, 65536);
     launch_h16_rank(st, a, c16, n);

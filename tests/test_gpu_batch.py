@@ -141,8 +141,8 @@ def test_autotune_restores_state_and_keeps_bits(sph):
                 b.upload(k, sc.pos, sc.vel)
             if tune:
                 t = b.autotune(3)
-                assert t["stencil_block"] in (64, 128, 256) and t["lane_walk"] in (0, 1)
-                assert t["trials"] == 6 and min(t["trial_ms"]) == t["ms_per_step"] > 0
+                assert t["stencil_block"] in (64, 128, 256) and t["lane_walk"] == 0
+                assert t["trials"] == 3 and min(t["trial_ms"][:3]) == t["ms_per_step"] > 0
             ms = b.step_timed(15)
             assert ms.shape == (15,) and np.all(ms > 0)
             res.append([_state(b.download(k)) for k in range(len(scenes))])

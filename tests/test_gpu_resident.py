@@ -191,12 +191,46 @@ def _blob_states(sph, sc, lane_walk, chunks=4, every=10):
     return out
 
 
-def test_union_walk_matches_lane_walk(sph):
-    """The wave-uniform candidate walk and the per-lane walk give
-    bit-identical states, step after step, on a clumped scene."""
-    sc = sph_scenes.blobs()
-    u = _blob_states(sph, sc, False)
-    l = _blob_states(sph, sc, True)
+def test_union_walk_matches_lane_walk_unsplit(sph):
+    """Where no wave task is split (every walk below the part size), the
+    wave-uniform candidate walk and the per-lane walk give bit-identical
+    states step after step."""
+    sc = sph_scenes.workload(0)
+    u = _blob_states(sph, sc, False, chunks=3, every=15)
+    l = _blob_states(sph, sc, True, chunks=3, every=15)
     for k, (x, y) in enumerate(zip(u, l)):
         for key in ("pos", "vel", "rho", "pressure", "ids"):
-            assert x[key].tobytes() == y[key].tobytes(), f"{key} differs after {10 * (k + 1)} steps"
+            assert x[key].tobytes() == y[key].tobytes(), f"{key} differs after {15 * (k + 1)} steps"
+
+
+def test_split_walks_deterministic_and_agree(sph, orc32):
+    """Clumped scene with split wave tasks: repeated runs are bit-identical;
+    the two walks agree within rounding; both match the oracle's densities."""
+    sc = sph_scenes.blobs()
+    a = _blob_states(sph, sc, False, chunks=2, every=5)
+    b = _blob_states(sph, sc, False, chunks=2, every=5)
+    for x, y in zip(a, b):
+        for key in ("pos", "vel", "rho", "pressure", "ids"):
+            assert x[key].tobytes() == y[key].tobytes()
+    s_g = sph.settings(*sc.settings_args())
+    res = []
+    for lane_walk in (False, True):
+        with sph.Context(sc.n, s_g, sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt,
+                         lane_walk=lane_walk) as ctx:
+            ctx.upload(sc.pos, sc.vel)
+            ctx.step(1)
+            d = ctx.download()
+            st = {}
+            for key in ("pos", "vel", "rho", "pressure"):
+                st[key] = np.zeros_like(d[key])
+                st[key][d["ids"].astype(np.int64)] = d[key]
+            res.append(st)
+    u, l = res
+    s_o = orc32.settings(*sc.settings_args())
+    ref, _, _, _, _ = pu.oracle_step(orc32, sc.pos, sc.vel, s_o, sc.dt,
+                                     orc32.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset))
+    pu.assert_rho_p(u["rho"], u["pressure"], ref["density"], ref["pressure"], s_o, "union walk")
+    pu.assert_rho_p(l["rho"], l["pressure"], ref["density"], ref["pressure"], s_o, "lane walk")
+    dv_ref = np.abs(l["vel"] - sc.vel).max(axis=1)
+    dv = np.abs(u["vel"] - l["vel"]).max(axis=1)
+    assert np.all(dv <= 1e-3 * dv_ref + 1e-5), float(np.max(dv - 1e-3 * dv_ref))
