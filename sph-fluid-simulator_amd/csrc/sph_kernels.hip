@@ -629,6 +629,36 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
     return 4ull * count <= 5ull * own_max;
 }
 
+// Scalar union walk: the positions [lo, hi) of the union's flattened
+// candidate sequence as wave-uniform runs (slots ascending), each split at
+// the wave's own slots [first, first + 64) so only that run carries the
+// self test.  run(j0, j1, self) walks slots [j0, j1); the candidate
+// records are then scalar loads broadcast to the lanes through SGPRs.
+template <class Run>
+__device__ inline void scalar_union_walk(const uint32_t *__restrict__ cstart, const UnionRows &U,
+                                         const StepParams &P, uint32_t lo, uint32_t hi, uint32_t first, Run run)
+{
+    uint32_t off = 0;
+    for (int z = U.z0; z <= U.z1; ++z)
+        for (int y = U.y0; y <= U.y1; ++y) {
+            const uint32_t r0 = row_base(P, y, z);
+            const uint32_t b = __builtin_amdgcn_readfirstlane(cstart[r0 + (uint32_t)U.x0]);
+            const uint32_t e = __builtin_amdgcn_readfirstlane(cstart[r0 + (uint32_t)U.x1 + 1u]);
+            const uint32_t s0 = max(lo, off), s1 = min(hi, off + (e - b));
+            if (s0 < s1) {
+                const uint32_t j0 = b + (s0 - off), j1 = b + (s1 - off);
+                const uint32_t m0 = min(max(j0, first), j1), m1 = min(max(j0, first + 64u), j1);
+                if (j0 < m0)
+                    run(j0, m0, std::false_type{});
+                if (m0 < m1)
+                    run(m0, m1, std::true_type{});
+                if (m1 < j1)
+                    run(m1, j1, std::false_type{});
+            }
+            off += e - b;
+        }
+}
+
 // Union walk staged through LDS.  The wave's union candidates, flattened
 // in row order (rows as for_each_row visits them, slots ascending), are
 // fetched 64 at a time -- one coalesced load per lane and record array, a
@@ -904,7 +934,6 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
     float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, SceneSet S,
     NeighborDump dump, PartPlan pp)
 {
-    __shared__ StageBuf<1> stage[BLOCK / 64];
     for_each_part<BLOCK>(S, pp, pp.queue, [&](uint32_t wt, uint32_t part, uint32_t nparts) {
         Lane L;
         bool active;
@@ -947,18 +976,29 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         UnionRows U;
         uint32_t ucount = 0;
         if (MODE == 0 && plan_union(cstart, cc, P, U, ucount) && !__any(slow)) {
-            // Wave-uniform candidate j, staged through LDS for the wave.
-            const float4 *const src[1] = {pos};
+            // Wave-uniform candidate j: scalar loads, four records at a time
+            // (density is too short per candidate to feed from LDS).
             const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
             const uint32_t hi = (uint32_t)((uint64_t)ucount * (part + 1u) / nparts);
-            staged_union_walk<1>(cstart, U, P, lo, hi, src, stage[threadIdx.x >> 6],
-                                 [&](uint32_t j, const float4(&r)[1]) {
-                const float4 q = r[0];
+            auto cand = [&](uint32_t j, float4 q, auto self_tag) {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
                 const float t = h2 - d2;
                 const float a = acc + mp6 * ((t * t) * t);  // (float)1 * term == term
-                acc = (d2 < h2 && j != s) ? a : acc;
+                acc = (d2 < h2 && (!decltype(self_tag)::value || j != s)) ? a : acc;
+            };
+            scalar_union_walk(cstart, U, P, lo, hi, L.first, [&](uint32_t j0, uint32_t j1, auto self_tag) {
+                const float4 *p = pos + j0;
+                uint32_t j = j0;
+                for (; j + 4u <= j1; j += 4u, p += 4) {
+                    const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+                    cand(j, q0, self_tag);
+                    cand(j + 1u, q1, self_tag);
+                    cand(j + 2u, q2, self_tag);
+                    cand(j + 3u, q3, self_tag);
+                }
+                for (; j < j1; ++j, ++p)
+                    cand(j, p[0], self_tag);
             });
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
