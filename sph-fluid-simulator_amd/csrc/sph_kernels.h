@@ -40,8 +40,9 @@ constexpr uint32_t kBigCellSortMax = 16384;  // larger cells keep arrival order
 constexpr int kBigSortBlock = 1024;        // threads of the big-cell sort
 constexpr int kBigSortBlocks = 256;        // blocks walking the big-cell list
 constexpr uint32_t kMaxParts = 16;         // parts of one split wave task, at most
-constexpr uint32_t kPartCandidates = 1024; // candidates per part, at least
+constexpr uint32_t kPartCandidates = 2048; // candidates per part, at least
 constexpr uint32_t kPartTasksPerWave = 4;  // part-task capacity per wave task
+constexpr uint32_t kLaneWalkWeight = 3;    // per-lane walk candidates count this many times
 
 struct ParticleArrays {
     float4 *posA, *velA, *posB, *velB;
@@ -72,8 +73,9 @@ struct SceneSet {
 };
 
 // Split wave tasks.  A wave task whose candidate walk is long (the union's
-// candidates, or the largest own stencil of a per-lane walk) runs as P
-// parts, each an equal slice of every lane's candidate sequence, with
+// candidates, or kLaneWalkWeight x the largest own stencil of a per-lane
+// walk) runs as P parts, each an equal slice of every lane's candidate
+// sequence, with
 // P = ceil(length / part size) up to kMaxParts; the part size is
 // kPartCandidates, raised when the scene's total walk would need more than
 // kPartTasksPerWave part tasks per wave task (a scene's split depends on

@@ -770,9 +770,12 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
         if (plan_union(cstart, cc, P, U, ucount) && !__any((cc & kSlowBit) != 0u)) {
             len = ucount;
         } else {
+            // Per-lane walks cost about three times as much per candidate
+            // as the wave-uniform walk (divergent gathers, no shared loads),
+            // so they count three times toward the part size.
             uint32_t own = 0;
             for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
-            len = (uint32_t)wave_max_i((int)own);
+            len = kLaneWalkWeight * (uint32_t)wave_max_i((int)own);
         }
         if ((threadIdx.x & 63u) == 0u) {
             measure[L.task] = len;
