@@ -39,10 +39,12 @@ constexpr uint32_t kSmallCell = 64;        // cells up to this size: stable rank
 constexpr uint32_t kBigCellSortMax = 16384;  // larger cells keep arrival order
 constexpr int kBigSortBlock = 1024;        // threads of the big-cell sort
 constexpr int kBigSortBlocks = 256;        // blocks walking the big-cell list
-constexpr uint32_t kMaxParts = 16;         // parts of one split wave task, at most
+constexpr uint32_t kPartBits = 6;          // part index bits of a part task
+constexpr uint32_t kMaxParts = 1u << kPartBits;  // parts of one split wave task, at most
 constexpr uint32_t kPartCandidates = 2048; // candidates per part, at least
 constexpr uint32_t kPartTasksPerWave = 4;  // part-task capacity per wave task
 constexpr uint32_t kLaneWalkWeight = 3;    // per-lane walk candidates count this many times
+constexpr uint32_t kSlowWalkWeight = 8;    // ... in a wave with slow lanes (m_ij counted per neighbour)
 
 struct ParticleArrays {
     float4 *posA, *velA, *posB, *velB;
@@ -92,7 +94,7 @@ struct PartPlan {
     unsigned long long *total;  // sum of the walk lengths per scene (zero between steps)
     uint32_t nscenes;
     uint32_t cap;        // part-task capacity (kPartTasksPerWave per wave task)
-    uint32_t *tasks;     // (wave task << 4) | part, cap entries
+    uint32_t *tasks;     // (wave task << kPartBits) | part, cap entries
     uint32_t *queue;     // pulled-task counters: [0] density, [1] force
     float4 *buf;         // partial sums: (part task) * 64 + lane, cap * 64 entries
     uint32_t *done;      // parts finished per wave task (zero between passes)

@@ -772,10 +772,13 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
         } else {
             // Per-lane walks cost about three times as much per candidate
             // as the wave-uniform walk (divergent gathers, no shared loads),
-            // so they count three times toward the part size.
+            // so they count three times toward the part size; with slow
+            // lanes (m_ij counted from the 27 buckets per neighbour, the
+            // wave waiting on them) more again.
             uint32_t own = 0;
             for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
-            len = kLaneWalkWeight * (uint32_t)wave_max_i((int)own);
+            const uint32_t weight = __any((cc & kSlowBit) != 0u) ? kSlowWalkWeight : kLaneWalkWeight;
+            len = weight * (uint32_t)wave_max_i((int)own);
         }
         if ((threadIdx.x & 63u) == 0u) {
             measure[L.task] = len;
@@ -837,7 +840,7 @@ __global__ void k_emit_parts(const uint32_t *__restrict__ off, uint32_t ntasks, 
         return;
     const uint32_t b = off[g], e = off[g + 1];
     for (uint32_t p = 0; b + p < e; ++p)
-        tasks[b + p] = (g << 4) | p;
+        tasks[b + p] = (g << kPartBits) | p;
 }
 
 void launch_plan_parts(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
@@ -881,7 +884,7 @@ __device__ inline void for_each_part(const SceneSet &S, const PartPlan &pp, uint
         if (t >= total)
             return;
         const uint32_t packed = __builtin_amdgcn_readfirstlane(pp.tasks[t]);
-        const uint32_t w = packed >> 4, part = packed & 15u;
+        const uint32_t w = packed >> kPartBits, part = packed & (kMaxParts - 1u);
         const uint32_t nparts = __builtin_amdgcn_readfirstlane(pp.off[w + 1] - pp.off[w]);
         task(w, part, nparts);
     }
@@ -986,9 +989,14 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             auto cand = [&](uint32_t j, float4 q, auto self_tag) {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                const float t = h2 - d2;
-                const float a = acc + mp6 * ((t * t) * t);  // (float)1 * term == term
-                acc = (d2 < h2 && (!decltype(self_tag)::value || j != s)) ? a : acc;
+                // Branch-free: h2 - d2 > 0 exactly when d2 < h2 (the float
+                // difference of unequal values is nonzero), so clamping it
+                // at 0 turns a failing candidate into a +0 term, which
+                // leaves acc unchanged bit for bit.  (float)1 * term == term.
+                float t = fmaxf(h2 - d2, 0.0f);
+                if (decltype(self_tag)::value)
+                    t = (j != s) ? t : 0.0f;
+                acc = acc + mp6 * ((t * t) * t);
             };
             scalar_union_walk(cstart, U, P, lo, hi, L.first, [&](uint32_t j0, uint32_t j1, auto self_tag) {
                 const float4 *p = pos + j0;
