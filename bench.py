@@ -231,6 +231,9 @@ def main():
     ap.add_argument("--serving-rates", default="50,500,2000", help="request rates (per second) to replay")
     ap.add_argument("--serving-window", type=float, default=5.0, help="seconds of arrivals per rate")
     ap.add_argument("--serving-budget", type=int, default=2 << 30, help="device bytes for admitted scenes")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the step's kernels directly instead of replaying a HIP graph (same kernels; "
+                         "for kernel tracers that do not follow graph replays)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -242,6 +245,7 @@ def main():
     s = sph_amd.settings(*sc.settings_args())
     b = sph_amd.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset)
     ctx = sph_amd.Context(sc.n, s, b, sc.dt)
+    ctx.set_graph(not args.no_graph)
     ctx.upload(sc.pos, sc.vel)
     ctx.step(args.warmup)
     ctx.sync()
