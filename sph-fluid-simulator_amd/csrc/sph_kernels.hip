@@ -577,8 +577,9 @@ __device__ inline uint32_t row_base(const StepParams &P, int y, int z)
 //    candidate index is wave-uniform and its record is a scalar load
 //    broadcast to every lane.
 // Union is taken when all lanes sit in one cell (the dense regime: one
-// cell holds many waves), or when their cells span at most 4 x 2 x 2 and
-// the union holds at most 5/4 of the largest own stencil.  A candidate the
+// cell holds many waves), or when their cells span at most 6 x 4 x 4 and
+// the union holds at most 3x the largest own stencil (a per-lane walk
+// costs about three times as much per candidate).  A candidate the
 // union adds to a lane's own stencil lies in a cell two or more cells away,
 // more than H >= h (1 + 2^-10) apart along some axis, so its float squared
 // distance is above h^2 and it never passes: both walks visit a lane's
@@ -605,7 +606,7 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
         y1 = wave_max_i(gy);
         z0 = wave_min_i(gz);
         z1 = wave_max_i(gz);
-        if (x1 - x0 > 3 || y1 - y0 > 1 || z1 - z0 > 1)
+        if (x1 - x0 > 5 || y1 - y0 > 3 || z1 - z0 > 3)
             return false;
     }
     U.x0 = x0 > 0 ? x0 - 1 : 0;
@@ -626,7 +627,7 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
     uint32_t own = 0;
     for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
     const uint32_t own_max = (uint32_t)wave_max_i((int)own);
-    return 4ull * count <= 5ull * own_max;
+    return count <= 3ull * own_max;
 }
 
 // Scalar union walk: the positions [lo, hi) of the union's flattened
@@ -681,7 +682,7 @@ __device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, co
 {
     const uint32_t lane = threadIdx.x & 63u;
     const int ny = U.y1 - U.y0 + 1;
-    const int nrows = ny * (U.z1 - U.z0 + 1);  // at most 4 x 4
+    const int nrows = ny * (U.z1 - U.z0 + 1);  // at most 6 x 6 <= 64 lanes
     uint32_t b = 0, len = 0;
     if ((int)lane < nrows) {
         const uint32_t r0 = row_base(P, U.y0 + (int)lane % ny, U.z0 + (int)lane / ny);
