@@ -14,6 +14,10 @@ namespace {
 // Device bytes per particle of a Workspace: posA, velA, posB, velB, posR,
 // velR (16 B each), cellcA, rankA, cellcB, perm, rhoB (4 B each).
 constexpr uint64_t kBytesPerParticle = 6 * 16 + 5 * 4;
+// Per 64-slot wave task, the part plan: part count, offset, walk length and
+// done counter (4 B each), then kPartTasksPerWave part-task entries (4 B)
+// and partial-sum records (64 lanes x 16 B).
+constexpr uint64_t kBytesPerWave = 16 + kPartTasksPerWave * (4 + 64 * 16);
 // Per grid cell: counts and cstart.
 constexpr uint64_t kBytesPerCell = 8;
 
@@ -47,7 +51,8 @@ double Server::now() const { return wall_seconds() - t0; }
 uint64_t Server::bytes_for(const BatchScene &s)
 {
     const uint64_t cells = (uint64_t)s.P.nx * s.P.ny * s.P.nz;
-    return kBytesPerParticle * s.desc.n + kBytesPerCell * cells + 4096;
+    const uint64_t waves = ((uint64_t)s.desc.n + 63u) / 64u;
+    return kBytesPerParticle * s.desc.n + kBytesPerWave * waves + kBytesPerCell * cells + 4096;
 }
 
 int Server::init(const sph_server_config_v1 *c)
@@ -72,7 +77,8 @@ int Server::init(const sph_server_config_v1 *c)
         cfg.max_steps_per_tick = 8;
     cfg.max_steps_per_tick = std::min<uint32_t>(cfg.max_steps_per_tick, 256u);
     batch.use_graph = (cfg.flags & SPH_SERVER_NO_GRAPH) == 0;
-    batch.max_capacity = std::min<uint64_t>(cfg.memory_budget / kBytesPerParticle, kMaxParticles);
+    batch.max_capacity =
+        std::min<uint64_t>(cfg.memory_budget / (kBytesPerParticle + kBytesPerWave / 64u), kMaxParticles);
     t0 = wall_seconds();
     return SPH_OK;
 }
