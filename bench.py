@@ -231,6 +231,9 @@ def main():
     ap.add_argument("--serving-rates", default="50,500,2000", help="request rates (per second) to replay")
     ap.add_argument("--serving-window", type=float, default=5.0, help="seconds of arrivals per rate")
     ap.add_argument("--serving-budget", type=int, default=2 << 30, help="device bytes for admitted scenes")
+    ap.add_argument("--no-reset", action="store_true",
+                    help="time the steps that follow the warm-up instead of restarting from the initial state "
+                         "(to profile late-job steps: --warmup 950 --steps 10)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying a HIP graph (same kernels; "
                          "for kernel tracers that do not follow graph replays)")
@@ -249,8 +252,9 @@ def main():
     ctx.upload(sc.pos, sc.vel)
     ctx.step(args.warmup)
     ctx.sync()
-    ctx.upload(sc.pos, sc.vel)  # the timed job starts from the initial state
-    ctx.sync()
+    if not args.no_reset:
+        ctx.upload(sc.pos, sc.vel)  # the timed job starts from the initial state
+        ctx.sync()
     t0 = time.perf_counter()
     ctx.step(args.steps)
     ctx.sync()
