@@ -15,7 +15,8 @@ The JSON line also carries
                 duration, timed with HIP events on the kernel's stream over
                 the same K-step job, against the 8 TB/s HBM peak; `traffic`
                 is the measured HBM bytes per launch when a counter summary
-                for the kernel is present under profiles/
+                for the kernel is present under profiles/; `copy_kernel_GBs`
+                the bandwidth a plain copy kernel reaches (read + write)
   cpu_baseline  the oracle (CPU restatement of the reference's
                 multithreaded update()) timed on this host over a few steps
                 of the same workload
@@ -286,9 +287,13 @@ def main():
         }
     ctx.close()
 
-    if args.copy_bytes > 0:
-        ms = sph_amd.copy_bench(args.copy_bytes, 10)
-        log(f"copy kernel: {args.copy_bytes} B in {ms:.4f} ms = {2 * args.copy_bytes / ms / 1e6:.1f} GB/s (read+write)")
+    # measured HBM bandwidth of a plain copy kernel (read + write), beside the 8 TB/s peak
+    copy_bytes = args.copy_bytes if args.copy_bytes > 0 else 1 << 30
+    ms = sph_amd.copy_bench(copy_bytes, 10)
+    copy_gbs = 2 * copy_bytes / ms / 1e6
+    log(f"copy kernel: {copy_bytes} B in {ms:.4f} ms = {copy_gbs:.1f} GB/s (read+write)")
+    if roofline is not None:
+        roofline["copy_kernel_GBs"] = round(copy_gbs, 1)
 
     cpu = None
     if not args.no_cpu:
