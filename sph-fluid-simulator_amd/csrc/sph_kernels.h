@@ -98,6 +98,9 @@ struct PartPlan {
     uint32_t *queue;     // pulled-task counters: [0] density, [1] force
     float4 *buf;         // partial sums: (part task) * 64 + lane, cap * 64 entries
     uint32_t *done;      // parts finished per wave task (zero between passes)
+    int4 *walk;          // per wave task, two entries: union box (x0, x1, y0, y1), (z0, z1, union
+                         // count, 1 = union walk / 0 = per-lane walk)
+    uint32_t *own;       // per slot of a per-lane-walk wave task: length of its own stencil
 };
 
 // Plans the parts of every wave task of S from the B-order cells and

@@ -630,6 +630,38 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
     return count <= 3ull * own_max;
 }
 
+// The walk a wave task takes and, for the union walk, its box and count:
+// from the step's part plan when there is one (decided once per wave task
+// by k_plan_parts), else decided here.  Waves with a slow lane take the
+// per-lane walk.
+__device__ inline bool walk_of(const PartPlan &pp, uint32_t wt, const uint32_t *__restrict__ cstart, uint32_t cc,
+                               const StepParams &P, bool slow, UnionRows &U, uint32_t &count)
+{
+    if (pp.walk) {
+        const int4 a = pp.walk[2u * wt], b = pp.walk[2u * wt + 1u];
+        U.x0 = __builtin_amdgcn_readfirstlane(a.x);
+        U.x1 = __builtin_amdgcn_readfirstlane(a.y);
+        U.y0 = __builtin_amdgcn_readfirstlane(a.z);
+        U.y1 = __builtin_amdgcn_readfirstlane(a.w);
+        U.z0 = __builtin_amdgcn_readfirstlane(b.x);
+        U.z1 = __builtin_amdgcn_readfirstlane(b.y);
+        count = (uint32_t)__builtin_amdgcn_readfirstlane(b.z);
+        return __builtin_amdgcn_readfirstlane(b.w) != 0;
+    }
+    return plan_union(cstart, cc, P, U, count) && !__any(slow);
+}
+
+// Length of the lane's own candidate sequence (its 3x3x3 stencil).
+__device__ inline uint32_t own_length(const PartPlan &pp, uint32_t s, const uint32_t *__restrict__ cstart, uint32_t cc,
+                                      const StepParams &P)
+{
+    if (pp.own)
+        return pp.own[s];
+    uint32_t own = 0;
+    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+    return own;
+}
+
 // Scalar union walk: the positions [lo, hi) of the union's flattened
 // candidate sequence as wave-uniform runs (slots ascending), each split at
 // the wave's own slots [first, first + 64) so only that run carries the
@@ -757,7 +789,8 @@ constexpr int kPlanBlock = 1024;
 __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__restrict__ cellcB,
                                                            const uint32_t *__restrict__ cstart, SceneSet S,
                                                            uint32_t *__restrict__ measure,
-                                                           unsigned long long *__restrict__ total)
+                                                           unsigned long long *__restrict__ total,
+                                                           int4 *__restrict__ walk, uint32_t *__restrict__ own_len)
 {
     __shared__ uint32_t s_scene[kPlanBlock / 64], s_len[kPlanBlock / 64];
     const uint32_t wave = threadIdx.x >> 6;
@@ -766,9 +799,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
     if (wave_lanes<kPlanBlock>(S, L, active)) {
         const StepParams &P = *L.P;
         const uint32_t cc = cellcB[L.slot];
-        UnionRows U;
+        UnionRows U{0, 0, 0, 0, 0, 0};
         uint32_t ucount = 0, len;
-        if (plan_union(cstart, cc, P, U, ucount) && !__any((cc & kSlowBit) != 0u)) {
+        const bool use_union = plan_union(cstart, cc, P, U, ucount) && !__any((cc & kSlowBit) != 0u);
+        if (use_union) {
             len = ucount;
         } else {
             // Per-lane walks cost about three times as much per candidate
@@ -780,6 +814,12 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
             for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
             const uint32_t weight = __any((cc & kSlowBit) != 0u) ? kSlowWalkWeight : kLaneWalkWeight;
             len = weight * (uint32_t)wave_max_i((int)own);
+            if (active)
+                own_len[L.slot] = own;
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            walk[2u * L.task] = make_int4(U.x0, U.x1, U.y0, U.y1);
+            walk[2u * L.task + 1u] = make_int4(U.z0, U.z1, (int)ucount, use_union ? 1 : 0);
         }
         if ((threadIdx.x & 63u) == 0u) {
             measure[L.task] = len;
@@ -848,7 +888,7 @@ void launch_plan_parts(hipStream_t st, const ParticleArrays &a, const CellArrays
                        const PartPlan &pp)
 {
     hipLaunchKernelGGL(k_plan_parts, dim3(blocks_for_waves(S.nwaves, kPlanBlock)), dim3(kPlanBlock), 0, st,
-                       a.cellcB, c.cstart, S, pp.measure, pp.total);
+                       a.cellcB, c.cstart, S, pp.measure, pp.total, pp.walk, pp.own);
     hipLaunchKernelGGL(k_count_parts, dim3(blocks_for(S.nwaves)), dim3(kBlock), 0, st, pp.measure, S, pp.total,
                        pp.nparts);
     CellArrays cs;
@@ -982,7 +1022,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         // task walks its slice [part, part + 1) / nparts of the sequence.
         UnionRows U;
         uint32_t ucount = 0;
-        if (MODE == 0 && plan_union(cstart, cc, P, U, ucount) && !__any(slow)) {
+        if (MODE == 0 && walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {
             // Wave-uniform candidate j: scalar loads, four records at a time
             // (density is too short per candidate to feed from LDS).
             const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
@@ -1015,8 +1055,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
             if (nparts > 1u) {
-                uint32_t own = 0;
-                for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+                const uint32_t own = own_length(pp, s, cstart, cc, P);
                 lo = (uint32_t)((uint64_t)own * part / nparts);
                 hi = (uint32_t)((uint64_t)own * (part + 1u) / nparts);
             }
@@ -1197,7 +1236,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
 
         UnionRows U;
         uint32_t ucount = 0;
-        if (plan_union(cstart, cc, P, U, ucount) && !__any(slow)) {  // as in k_density
+        if (walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {  // as in k_density
             const float4 *const src[2] = {posR, velR};
             const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
             const uint32_t hi = (uint32_t)((uint64_t)ucount * (part + 1u) / nparts);
@@ -1212,8 +1251,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
             if (nparts > 1u) {
-                uint32_t own = 0;
-                for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+                const uint32_t own = own_length(pp, s, cstart, cc, P);
                 lo = (uint32_t)((uint64_t)own * part / nparts);
                 hi = (uint32_t)((uint64_t)own * (part + 1u) / nparts);
             }

@@ -46,7 +46,7 @@ struct Workspace {
     // split wave tasks: parts per task (padded), their scan, scan partials,
     // walk lengths and their total, part tasks, queue counters, partial sums,
     // finished-part counters
-    DevBuf pnparts, poff, ppartials, pmeasure, ptotal, ptasks, pqueue, pbuf, pdone;
+    DevBuf pnparts, poff, ppartials, pmeasure, ptotal, ptasks, pqueue, pbuf, pdone, pwalk, pown;
     uint32_t parts_padded = 0;
     uint32_t ncells_padded = 0;
     uint32_t big_cap = 0;

@@ -263,6 +263,9 @@ int Workspace::set_scenes(const std::vector<StepParams> &scenes, hipStream_t st)
         (rc = tasks.reserve(sizeof(WaveTask) * std::max<size_t>(t.size(), 1))))
         return rc;
     const size_t nw = std::max<size_t>(t.size(), 1);
+    size_t nslots = 0;
+    for (const StepParams &sp : scenes)
+        nslots = std::max<size_t>(nslots, sp.p_end);
     const uint32_t padded = (uint32_t)((nw + kScanChunk - 1) / kScanChunk * kScanChunk);
     const size_t cap = kPartTasksPerWave * nw;
     const bool fresh_counters = padded > parts_padded || pdone.bytes < 4 * nw ||
@@ -271,7 +274,8 @@ int Workspace::set_scenes(const std::vector<StepParams> &scenes, hipStream_t st)
         (rc = ppartials.reserve(4ull * (padded / kScanChunk))) || (rc = pmeasure.reserve(4ull * nw)) ||
         (rc = ptotal.reserve(8ull * std::max<size_t>(scenes.size(), 1))) || (rc = ptasks.reserve(4ull * cap)) ||
         (rc = pqueue.reserve(16)) ||
-        (rc = pbuf.reserve(sizeof(float4) * 64ull * cap)) || (rc = pdone.reserve(4ull * nw)))
+        (rc = pbuf.reserve(sizeof(float4) * 64ull * cap)) || (rc = pdone.reserve(4ull * nw)) ||
+        (rc = pwalk.reserve(2ull * sizeof(int4) * nw)) || (rc = pown.reserve(4ull * std::max<size_t>(nslots, 1))))
         return rc;
     if (fresh_counters) {  // zero outside use from here on: the step keeps them so
         launch_fill_u32(st, (uint32_t *)pnparts.ptr, 0u, pnparts.bytes / 4);
@@ -347,6 +351,8 @@ PartPlan Workspace::part_plan() const
     p.queue = (uint32_t *)pqueue.ptr;
     p.buf = (float4 *)pbuf.ptr;
     p.done = (uint32_t *)pdone.ptr;
+    p.walk = (int4 *)pwalk.ptr;
+    p.own = (uint32_t *)pown.ptr;
     return p;
 }
 
@@ -354,7 +360,7 @@ void Workspace::release_all()
 {
     for (DevBuf *b : {&posA, &velA, &posB, &velB, &cellcA, &rankA, &cellcB, &perm, &rhoB, &posR, &velR,
                       &counts, &cstart, &partials, &big, &force, &params, &tasks, &pnparts, &poff,
-                      &ppartials, &pmeasure, &ptotal, &ptasks, &pqueue, &pbuf, &pdone})
+                      &ppartials, &pmeasure, &ptotal, &ptasks, &pqueue, &pbuf, &pdone, &pwalk, &pown})
         b->release();
     ncells_padded = 0;
     big_cap = 0;

@@ -12,12 +12,13 @@ namespace sph {
 namespace {
 
 // Device bytes per particle of a Workspace: posA, velA, posB, velB, posR,
-// velR (16 B each), cellcA, rankA, cellcB, perm, rhoB (4 B each).
-constexpr uint64_t kBytesPerParticle = 6 * 16 + 5 * 4;
+// velR (16 B each), cellcA, rankA, cellcB, perm, rhoB and the part plan's
+// own-stencil length (4 B each).
+constexpr uint64_t kBytesPerParticle = 6 * 16 + 6 * 4;
 // Per 64-slot wave task, the part plan: part count, offset, walk length and
-// done counter (4 B each), then kPartTasksPerWave part-task entries (4 B)
-// and partial-sum records (64 lanes x 16 B).
-constexpr uint64_t kBytesPerWave = 16 + kPartTasksPerWave * (4 + 64 * 16);
+// done counter (4 B each), the walk (32 B), then kPartTasksPerWave part-task
+// entries (4 B) and partial-sum records (64 lanes x 16 B).
+constexpr uint64_t kBytesPerWave = 16 + 32 + kPartTasksPerWave * (4 + 64 * 16);
 // Per grid cell: counts and cstart.
 constexpr uint64_t kBytesPerCell = 8;
 
