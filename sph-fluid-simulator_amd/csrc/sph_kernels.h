@@ -39,10 +39,13 @@ constexpr uint32_t kSmallCell = 64;        // cells up to this size: stable rank
 constexpr uint32_t kBigCellSortMax = 16384;  // larger cells keep arrival order
 constexpr int kBigSortBlock = 1024;        // threads of the big-cell sort
 constexpr int kBigSortBlocks = 256;        // blocks walking the big-cell list
-constexpr uint32_t kPartBits = 6;          // part index bits of a part task
+constexpr uint32_t kPartBits = 8;          // part index bits of a part task
 constexpr uint32_t kMaxParts = 1u << kPartBits;  // parts of one split wave task, at most
 constexpr uint32_t kPartCandidates = 2048; // candidates per part, at least
-constexpr uint32_t kPartTasksPerWave = 4;  // part-task capacity per wave task
+constexpr uint32_t kPartRoomPerWave = 3;   // spare part tasks per wave task of a scene, at least
+constexpr uint32_t kPartRoomMin = 49152;   // ... raised to this many per scene (~10 rounds of the
+                                           // resident waves, so tasks stay short in mid-size scenes)
+constexpr uint32_t kPartRoomMaxPerWave = 16;  // ... but at most this many per wave task
 constexpr uint32_t kLaneWalkWeight = 3;    // per-lane walk candidates count this many times
 constexpr uint32_t kSlowWalkWeight = 8;    // ... in a wave with slow lanes (m_ij counted per neighbour)
 
@@ -74,14 +77,24 @@ struct SceneSet {
     uint32_t nwaves;
 };
 
+// Spare part tasks of a scene of `waves` wave tasks (beyond one per task):
+// the scene's part size is its total walk over this room, so a scene
+// splits into at most waves + part_room(waves) part tasks.  Depends on the
+// scene alone.
+SPH_HD inline uint64_t part_room(uint64_t waves)
+{
+    const uint64_t lo = kPartRoomPerWave * waves, hi = kPartRoomMaxPerWave * waves;
+    const uint64_t want = hi < kPartRoomMin ? hi : (uint64_t)kPartRoomMin;
+    return lo > want ? lo : want;
+}
+
 // Split wave tasks.  A wave task whose candidate walk is long (the union's
 // candidates, or kLaneWalkWeight x the largest own stencil of a per-lane
 // walk) runs as P parts, each an equal slice of every lane's candidate
-// sequence, with
-// P = ceil(length / part size) up to kMaxParts; the part size is
-// kPartCandidates, raised when the scene's total walk would need more than
-// kPartTasksPerWave part tasks per wave task (a scene's split depends on
-// the scene alone, so it is the same in any batch).  Parts are separate tasks pulled from a queue; the
+// sequence, with P = ceil(length / part size) up to kMaxParts; the part size
+// is kPartCandidates, raised when the scene's total walk over part_room
+// would exceed it (a scene's split depends on the scene alone, so it is the
+// same in any batch).  Parts are separate tasks pulled from a queue; the
 // last one to finish adds the partial sums in part order, so results are
 // deterministic.  An unsplit task sums in candidate order (and its two
 // walks agree bit for bit); a split one differs from it by rounding only.
@@ -93,7 +106,7 @@ struct PartPlan {
     uint32_t *measure;   // walk length per wave task
     unsigned long long *total;  // sum of the walk lengths per scene (zero between steps)
     uint32_t nscenes;
-    uint32_t cap;        // part-task capacity (kPartTasksPerWave per wave task)
+    uint32_t cap;        // part-task capacity: sum over scenes of waves + part_room(waves)
     uint32_t *tasks;     // (wave task << kPartBits) | part, cap entries
     uint32_t *queue;     // pulled-task counters: [0] density, [1] force
     float4 *buf;         // partial sums: (part task) * 64 + lane, cap * 64 entries

@@ -849,8 +849,7 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
 }
 
 // Parts per wave task: the part size grows beyond kPartCandidates when the
-// scene's total walk would need more than kPartTasksPerWave part tasks per
-// wave task of the scene.
+// scene's total walk over its part room would exceed it.
 __global__ void k_count_parts(const uint32_t *__restrict__ measure, SceneSet S,
                               const unsigned long long *__restrict__ total, uint32_t *__restrict__ nparts)
 {
@@ -860,7 +859,7 @@ __global__ void k_count_parts(const uint32_t *__restrict__ measure, SceneSet S,
     const uint32_t scene = S.tasks[g].scene;
     const StepParams &P = S.params[scene];
     const unsigned long long waves = (P.p_end - P.p_begin + 63u) / 64u;
-    const unsigned long long room = (kPartTasksPerWave - 1u) * waves;
+    const unsigned long long room = part_room(waves);
     const unsigned long long size = max((unsigned long long)kPartCandidates, (total[scene] + room - 1u) / room);
     const unsigned long long p = (measure[g] + size - 1u) / size;
     nparts[g] = (uint32_t)min(max(p, 1ull), (unsigned long long)kMaxParts);

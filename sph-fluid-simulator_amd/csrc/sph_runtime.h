@@ -52,6 +52,7 @@ struct Workspace {
     uint32_t big_cap = 0;
     uint32_t nwaves = 0;
     uint32_t nscenes = 0;
+    uint32_t part_cap = 0;  // part-task capacity of the current scene set
     int stencil_block = kStencilBlock;  // threads per block of the density and force passes
     int reserve_particles(uint32_t n);
     int reserve_cells(uint64_t ncells);

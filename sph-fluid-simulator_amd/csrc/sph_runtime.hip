@@ -267,7 +267,14 @@ int Workspace::set_scenes(const std::vector<StepParams> &scenes, hipStream_t st)
     for (const StepParams &sp : scenes)
         nslots = std::max<size_t>(nslots, sp.p_end);
     const uint32_t padded = (uint32_t)((nw + kScanChunk - 1) / kScanChunk * kScanChunk);
-    const size_t cap = kPartTasksPerWave * nw;
+    size_t cap = 0;  // every scene's wave tasks plus its part room
+    for (const StepParams &sp : scenes) {
+        const uint64_t waves = ((uint64_t)sp.p_end - sp.p_begin + 63u) / 64u;
+        cap += waves + part_room(waves);
+    }
+    cap = std::max<size_t>(cap, nw);
+    if (cap > 0xFFFFFFFFull)
+        return fail(SPH_E_LIMIT, "part plan: %zu part tasks", cap);
     const bool fresh_counters = padded > parts_padded || pdone.bytes < 4 * nw ||
                                 ptotal.bytes < 8 * std::max<size_t>(scenes.size(), 1);
     if ((rc = pnparts.reserve(4ull * padded)) || (rc = poff.reserve(4ull * (padded + 1))) ||
@@ -295,6 +302,7 @@ int Workspace::set_scenes(const std::vector<StepParams> &scenes, hipStream_t st)
         return hip_fail(e, "scene table upload");
     nwaves = (uint32_t)t.size();
     nscenes = (uint32_t)scenes.size();
+    part_cap = (uint32_t)cap;
     return SPH_OK;
 }
 
@@ -346,7 +354,7 @@ PartPlan Workspace::part_plan() const
     p.measure = (uint32_t *)pmeasure.ptr;
     p.total = (unsigned long long *)ptotal.ptr;
     p.nscenes = nscenes;
-    p.cap = kPartTasksPerWave * std::max(nwaves, 1u);
+    p.cap = std::max(part_cap, 1u);
     p.tasks = (uint32_t *)ptasks.ptr;
     p.queue = (uint32_t *)pqueue.ptr;
     p.buf = (float4 *)pbuf.ptr;

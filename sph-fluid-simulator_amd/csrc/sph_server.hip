@@ -16,9 +16,11 @@ namespace {
 // own-stencil length (4 B each).
 constexpr uint64_t kBytesPerParticle = 6 * 16 + 6 * 4;
 // Per 64-slot wave task, the part plan: part count, offset, walk length and
-// done counter (4 B each), the walk (32 B), then kPartTasksPerWave part-task
-// entries (4 B) and partial-sum records (64 lanes x 16 B).
-constexpr uint64_t kBytesPerWave = 16 + 32 + kPartTasksPerWave * (4 + 64 * 16);
+// done counter (4 B each) and the walk (32 B); per part task (one per wave
+// task plus the scene's part room) its entry (4 B) and partial-sum records
+// (64 lanes x 16 B).
+constexpr uint64_t kBytesPerWave = 16 + 32;
+constexpr uint64_t kBytesPerPartTask = 4 + 64 * 16;
 // Per grid cell: counts and cstart.
 constexpr uint64_t kBytesPerCell = 8;
 
@@ -53,7 +55,8 @@ uint64_t Server::bytes_for(const BatchScene &s)
 {
     const uint64_t cells = (uint64_t)s.P.nx * s.P.ny * s.P.nz;
     const uint64_t waves = ((uint64_t)s.desc.n + 63u) / 64u;
-    return kBytesPerParticle * s.desc.n + kBytesPerWave * waves + kBytesPerCell * cells + 4096;
+    return kBytesPerParticle * s.desc.n + kBytesPerWave * waves + kBytesPerPartTask * (waves + part_room(waves)) +
+           kBytesPerCell * cells + 4096;
 }
 
 int Server::init(const sph_server_config_v1 *c)
@@ -79,7 +82,8 @@ int Server::init(const sph_server_config_v1 *c)
     cfg.max_steps_per_tick = std::min<uint32_t>(cfg.max_steps_per_tick, 256u);
     batch.use_graph = (cfg.flags & SPH_SERVER_NO_GRAPH) == 0;
     batch.max_capacity =
-        std::min<uint64_t>(cfg.memory_budget / (kBytesPerParticle + kBytesPerWave / 64u), kMaxParticles);
+        std::min<uint64_t>(cfg.memory_budget / (kBytesPerParticle + (kBytesPerWave + kBytesPerPartTask) / 64u),
+                           kMaxParticles);
     t0 = wall_seconds();
     return SPH_OK;
 }
