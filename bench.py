@@ -16,7 +16,10 @@ The JSON line also carries
                 the same K-step job, against the 8 TB/s HBM peak; `traffic`
                 is the measured HBM bytes per launch when a counter summary
                 for the kernel is present under profiles/; `copy_kernel_GBs`
-                the bandwidth a plain copy kernel reaches (read + write)
+                the bandwidth a plain copy kernel reaches (read + write);
+                `per_kernel_alg_GBs` every pass's algorithmic GB/s and
+                `l2_hit` the per-pass L2 hit rates of the committed counter
+                summary (early and late in the job)
   cpu_baseline  the oracle (CPU restatement of the reference's
                 multithreaded update()) timed on this host over a few steps
                 of the same workload
@@ -124,6 +127,16 @@ def load_traffic(kernel):
         return None
     with open(path) as f:
         return json.load(f).get("hbm_bytes_per_launch")
+
+
+def load_l2_hit():
+    """Per-pass L2 hit rates (early and late in the job) from a committed
+    counter summary (profiles/l2_hit_workload2.json), or None."""
+    path = os.path.join(REPO, "profiles", "l2_hit_workload2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hit_rate")
 
 
 def dropin_leg(sc, calls=5):
@@ -284,6 +297,10 @@ def main():
             "step_alg_GBs": round(ALG_BYTES_STEP * value / 1e9, 2),
             "step_frac": round(ALG_BYTES_STEP * value / 1e9 / HBM_PEAK_GBS, 4),
             "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()},
+            # each pass's algorithmic bytes per launch over its mean duration
+            "per_kernel_alg_GBs": {k: round(ALG_BYTES[k] * sc.n / (v * 1e-3) / 1e9, 2) for k, v in prof.items()
+                                   if k in ALG_BYTES and v > 0},
+            "l2_hit": load_l2_hit(),
         }
     ctx.close()
 
