@@ -1093,9 +1093,15 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
 
 // Grid of a stencil pass: one wave per wave task, or with a part plan as
 // many waves as the device holds at once (they pull the part tasks).
+// Waves of a stencil pass: one per wave task without a plan; with one, a
+// persistent grid of at most the resident waves and 4 per wave task (more
+// would only pull from an empty queue: a small scene splits into about as
+// many parts as it has wave tasks).
 inline uint32_t stencil_waves(const SceneSet &S, const PartPlan *pp)
 {
-    return pp ? std::min<uint32_t>(pp->cap, resident_waves()) : S.nwaves;
+    if (!pp)
+        return S.nwaves;
+    return std::min<uint32_t>(std::min<uint32_t>(pp->cap, resident_waves()), 4u * std::max(S.nwaves, 1u));
 }
 
 template <int MODE, int BLOCK>
