@@ -916,6 +916,16 @@ __device__ inline void for_each_part(const SceneSet &S, const PartPlan &pp, uint
         return;
     }
     const uint32_t total = __builtin_amdgcn_readfirstlane(pp.off[S.nwaves]);
+    if (total == S.nwaves) {
+        // No task is split this step (uniform, short walks): a static,
+        // XCD-contiguous assignment -- wave v takes tasks v, v + G, ... --
+        // with no queue traffic, neighbouring tasks sharing an L2.
+        const uint32_t G = gridDim.x * (BLOCK / 64);
+        for (uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
+             w < S.nwaves; w += G)
+            task(w, 0u, 1u);
+        return;
+    }
     for (;;) {
         uint32_t t = 0;
         if ((threadIdx.x & 63u) == 0u)
@@ -1091,17 +1101,27 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
     });
 }
 
-// Grid of a stencil pass: one wave per wave task, or with a part plan as
-// many waves as the device holds at once (they pull the part tasks).
+// Waves of `kernel` (at `block` threads) the device holds at once, from its
+// register and LDS use.
+static uint32_t occupancy_waves(const void *kernel, int block)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb <= 0)
+        return resident_waves();
+    return std::min<uint32_t>(resident_waves(), (uint32_t)nb * (uint32_t)(block / 64) * (resident_waves() / 32u));
+}
+
 // Waves of a stencil pass: one per wave task without a plan; with one, a
-// persistent grid of at most the resident waves and 4 per wave task (more
-// would only pull from an empty queue: a small scene splits into about as
-// many parts as it has wave tasks).
-inline uint32_t stencil_waves(const SceneSet &S, const PartPlan *pp)
+// persistent grid of the waves the device holds at once for this kernel
+// (a wave launched beyond them would start only as the first ones finish,
+// with a full share of a static assignment), and at most 4 per wave task
+// (more would only pull from an empty queue: a small scene splits into
+// about as many parts as it has wave tasks).
+inline uint32_t stencil_waves(const SceneSet &S, const PartPlan *pp, uint32_t resident)
 {
     if (!pp)
         return S.nwaves;
-    return std::min<uint32_t>(std::min<uint32_t>(pp->cap, resident_waves()), 4u * std::max(S.nwaves, 1u));
+    return std::min<uint32_t>(std::min<uint32_t>(pp->cap, resident), 4u * std::max(S.nwaves, 1u));
 }
 
 template <int MODE, int BLOCK>
@@ -1111,7 +1131,8 @@ static void launch_density_block(hipStream_t st, const ParticleArrays &a, const 
     PartPlan p{};
     if (pp)
         p = *pp;
-    hipLaunchKernelGGL((k_density<MODE, BLOCK>), dim3(std::max(1u, blocks_for_waves(stencil_waves(S, pp), BLOCK))),
+    static const uint32_t resident = occupancy_waves(reinterpret_cast<const void *>(&k_density<MODE, BLOCK>), BLOCK);
+    hipLaunchKernelGGL((k_density<MODE, BLOCK>), dim3(std::max(1u, blocks_for_waves(stencil_waves(S, pp, resident), BLOCK))),
                        dim3(BLOCK), 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB, a.posR, a.velR, S, d, p);
 }
 
@@ -1324,7 +1345,9 @@ static void launch_force_block(hipStream_t st, const ParticleArrays &a, const Ce
     PartPlan p{};
     if (pp)
         p = *pp;
-    dim3 g(std::max(1u, blocks_for_waves(stencil_waves(S, pp), BLOCK))), b(BLOCK);
+    static const uint32_t resident =
+        occupancy_waves(reinterpret_cast<const void *>(&k_force_integrate<false, true, BLOCK>), BLOCK);
+    dim3 g(std::max(1u, blocks_for_waves(stencil_waves(S, pp, resident), BLOCK))), b(BLOCK);
 #define SPH_LAUNCH_FI(WF, NK)                                                                          \
     hipLaunchKernelGGL((k_force_integrate<WF, NK, BLOCK>), g, b, 0, st, a.posB, a.velB, a.posR, a.velR, \
                        a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,         \
