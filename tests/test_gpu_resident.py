@@ -138,7 +138,8 @@ def test_drift_report_workload0(sph, orc32, orc64):
     """1000 free-running steps of workload 0: GPU FP32 and oracle FP32 each
     against the oracle in FP64 (same method).  The system is chaotic, so the
     contract is that the GPU drifts no faster than the reference's own FP32
-    arithmetic; the numbers are written to gpurun_out/drift_workload0.json."""
+    arithmetic.  The report is printed, and written to
+    $SPH_REPORT_DIR/drift_workload0.json when that directory is set."""
     sc = sph_scenes.workload(0)
     checkpoints = [1, 10, 100, 1000]
     s32 = orc32.settings(*sc.settings_args())
@@ -169,8 +170,8 @@ def test_drift_report_workload0(sph, orc32, orc64):
                 "cpu_fp32_vs_fp64": {"max": float(dc.max()), "rms": float(np.sqrt((dc ** 2).mean())),
                                      "p99": float(np.percentile(dc, 99))},
             }
-    out_dir = os.path.join(REPO, "gpurun_out")
-    if os.path.isdir(out_dir):
+    out_dir = os.environ.get("SPH_REPORT_DIR", "")
+    if out_dir and os.path.isdir(out_dir):
         with open(os.path.join(out_dir, "drift_workload0.json"), "w") as f:
             json.dump(report, f, indent=1)
     print(json.dumps(report))
