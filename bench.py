@@ -69,6 +69,20 @@ def cpu_threads():
     return min(os.cpu_count() or 1, 16)
 
 
+def cpu_model():
+    """The host CPU's model name and its logical CPU count, as lscpu reports them."""
+    name = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs visible"
+
+
 def job_states(sc, steps, nphases=5):
     """States of the workload's job at evenly spaced steps (0 .. steps-1),
     taken from the GPU run: the step cost grows as the fluid clumps, so the
@@ -114,8 +128,7 @@ def cpu_baseline(sc, states, budget_s=30.0):
         "kind": "port",
         "sample": f"one oracle step from each of the job's states at steps {used} (GPU snapshots of the "
                   f"{sc.n}-particle workload), mean {mean * 1e3:.1f} ms/step, per phase "
-                  f"{[round(t * 1e3, 1) for t in times]} ms, {threads} threads, "
-                  f"{platform.processor() or platform.machine()}",
+                  f"{[round(t * 1e3, 1) for t in times]} ms, {threads} threads on {cpu_model()}",
     }
 
 
