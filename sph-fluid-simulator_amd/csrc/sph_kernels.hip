@@ -66,11 +66,9 @@ __device__ inline bool lane_of(const SceneSet &S, Lane &out)
     return out.slot < out.P->p_end;
 }
 
-// As lane_of, but every lane of a live wave stays live: lanes past the
-// scene's last slot mirror the wave's first slot (always a valid slot) and
-// report active = false, so wave-wide votes and shuffles see defined data.
-// Returns false only for waves past the task list (the whole wave).
-// Wave task w (wave-uniform) with every lane live (see wave_lanes).
+// Wave task w (wave-uniform) with every lane live: lanes past the scene's
+// last slot mirror the wave's first slot (always a valid slot) and report
+// active = false, so wave-wide votes and shuffles see defined data.
 __device__ inline void wave_task(const SceneSet &S, uint32_t w, Lane &out, bool &active)
 {
     const WaveTask t = S.tasks[w];
@@ -84,6 +82,8 @@ __device__ inline void wave_task(const SceneSet &S, uint32_t w, Lane &out, bool 
     out.slot = active ? slot : first;
 }
 
+// As lane_of, but with every lane of a live wave live (see wave_task).
+// Returns false only for waves past the task list (the whole wave).
 template <int BLOCK>
 __device__ inline bool wave_lanes(const SceneSet &S, Lane &out, bool &active)
 {
