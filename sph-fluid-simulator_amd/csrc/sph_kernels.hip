@@ -991,6 +991,16 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
     NeighborDump dump, PartPlan pp)
 {
     for_each_part<BLOCK>(S, pp, pp.queue, [&](uint32_t wt, uint32_t part, uint32_t nparts) {
+        // The density pass takes the plan's parts kDensityPartGroup at a time
+        // (its work per candidate is a fraction of the force pass's, so its
+        // tasks would be short next to their set-up): part p = g k walks the
+        // slice of parts [p, p + g) as density part k of ceil(nparts / g);
+        // the other parts are skipped.
+        constexpr uint32_t g = kDensityPartGroup;
+        if (part % g)
+            return;
+        const uint32_t part_end = min(part + g, nparts);
+        const uint32_t dpart = part / g, dparts = (nparts + g - 1u) / g;
         Lane L;
         bool active;
         wave_task(S, wt, L, active);
@@ -1028,14 +1038,14 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         };
         // Waves with a slow-multiplicity lane (rare where the union pays)
         // take the per-lane walk, so the union walk needs no m_ij.  A split
-        // task walks its slice [part, part + 1) / nparts of the sequence.
+        // task walks its slice [part, part_end) / nparts of the sequence.
         UnionRows U;
         uint32_t ucount = 0;
         if (MODE == 0 && walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {
             // Wave-uniform candidate j: scalar loads, four records at a time
             // (density is too short per candidate to feed from LDS).
             const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
-            const uint32_t hi = (uint32_t)((uint64_t)ucount * (part + 1u) / nparts);
+            const uint32_t hi = (uint32_t)((uint64_t)ucount * part_end / nparts);
             auto cand = [&](uint32_t j, float4 q, auto self_tag) {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
@@ -1063,10 +1073,10 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             });
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
-            if (nparts > 1u) {
+            if (dparts > 1u) {
                 const uint32_t own = own_length(pp, s, cstart, cc, P);
                 lo = (uint32_t)((uint64_t)own * part / nparts);
-                hi = (uint32_t)((uint64_t)own * (part + 1u) / nparts);
+                hi = (uint32_t)((uint64_t)own * part_end / nparts);
             }
             for_each_row_span(cstart, cc, P, lo, hi, [&](uint32_t b, uint32_t e) {
                 uint32_t j = b;
@@ -1082,9 +1092,9 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                     body(j, pos[j]);
             });
         }
-        if (nparts > 1u) {
+        if (dparts > 1u) {
             float v[1] = {acc};
-            if (!combine_parts<1>(pp, wt, part, nparts, v))
+            if (!combine_parts<1>(pp, wt, dpart, dparts, v))
                 return;
             acc = v[0];
         }
