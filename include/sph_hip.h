@@ -77,7 +77,10 @@ typedef struct sph_scene_desc_v1 {
     float dt;                 /* time step used by sph_hip_step */
     sph_settings_v1 settings; /* from sph_settings_init */
     sph_box_v1 box;
-    float cell_scale;         /* grid cell edge in units of h; 0 or 1 = 1 (min) */
+    float cell_scale;         /* grid cell edge in units of h (1 + 2^-10): 0 = 1; >= 1
+                                 coarser cells; in [1/3, 1) finer cells with a stencil of
+                                 ceil(1 / cell_scale) cells each way (fewer candidates per
+                                 particle, more cells); results agree to rounding */
     uint32_t flags;           /* SPH_SCENE_* bits, others must be 0 */
 } sph_scene_desc_v1;
 
@@ -202,12 +205,21 @@ int sph_batch_step_timed(sph_batch *b, int nsteps, float *ms);
  * after the last, and keeps the fastest for the following steps.  The knob
  * changes no result bit.  trial_ms[k] is the mean step time of trial k
  * (k = 0, 1, 2 for 64, 128, 256 threads; 3-5 unused).  The reference has no
- * counterpart (its thread count is hardware_concurrency, src/sph.cpp:199). */
+ * counterpart (its thread count is hardware_concurrency, src/sph.cpp:199).
+ *
+ * The _grid variants also tune the grid's cell size multiple: every scene
+ * is re-gridded with cell scale 1 (trials 0-2) and 1/2 (trials 3-5, a
+ * 5x5x5 stencil of half-size cells: about half the candidates where the
+ * density is even at the scale of h, more rows where particles cluster
+ * tighter than h), and the fastest pair is kept, also for later uploads
+ * (it replaces the scenes' own cell_scale).  The grid changes the
+ * order of each particle's sum, so results change by rounding (each scene
+ * stays bit-identical to the same scene stepped alone on that grid). */
 typedef struct sph_tune_v1 {
     int32_t stencil_block;       /* chosen threads per block */
     int32_t lane_walk;           /* always 0 (reserved) */
     int32_t trials;
-    int32_t _pad;
+    float cell_scale;            /* _grid variants: chosen cell scale (1 or 0.5), else 0 */
     double ms_per_step;          /* mean step time of the chosen setting */
     double default_ms_per_step;  /* same for 128 threads */
     double trial_ms[6];
@@ -215,6 +227,8 @@ typedef struct sph_tune_v1 {
 
 int sph_batch_autotune(sph_batch *b, int trial_steps, sph_tune_v1 *out);
 int sph_hip_autotune(sph_ctx *ctx, int trial_steps, sph_tune_v1 *out);
+int sph_batch_autotune_grid(sph_batch *b, int trial_steps, sph_tune_v1 *out);
+int sph_hip_autotune_grid(sph_ctx *ctx, int trial_steps, sph_tune_v1 *out);
 
 /* ---- serving loop -----------------------------------------------------------
  * Continuous batching of scene-advance requests.  No counterpart in the

@@ -14,11 +14,16 @@ constexpr uint32_t kMaxParticles = 1u << 30;
 struct BatchScene {
     sph_scene_desc_v1 desc;
     StepParams P;  // constants, grid, slot range
+    double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};  // bounds the grid covers
     bool uploaded = false;
 };
 
 int validate_desc(const sph_scene_desc_v1 &d);
-void scene_grid(StepParams &P, const sph_scene_desc_v1 &d, const float *pos, uint32_t n);
+// Bounds of b's grid from its box and particles, then its grid (scene_regrid
+// with the scene's own cell scale).
+void scene_grid(BatchScene &b, const float *pos, uint32_t n);
+// b's grid over its bounds with cell scale `scale` (0: desc.cell_scale).
+int scene_regrid(BatchScene &b, double scale);
 // A scene entry for desc (validated): constants, flags; no slot range yet.
 int make_scene(const sph_scene_desc_v1 &d, BatchScene &out);
 
@@ -60,7 +65,8 @@ struct Batch {
     // Times trial_steps steps from the current state for every stencil
     // block size (64, 128, 256), restoring the state before each trial and
     // after the last, and keeps the fastest.  Results do not depend on it.
-    int autotune(int trial_steps, sph_tune_v1 *out);
+    // grid: also try cell scales 1 and 1/2 for every scene (changes results by rounding)
+    int autotune(int trial_steps, sph_tune_v1 *out, bool grid = false);
     int sync();
     int download(uint32_t s, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids);
     int download_particles(uint32_t s, sph_particle_v1 *out, float *transforms);

@@ -38,16 +38,17 @@ int validate_desc(const sph_scene_desc_v1 &d)
         return rc;
     if (!std::isfinite(d.dt))
         return fail(SPH_E_INVALID, "scene: dt must be finite");
-    if (!(d.cell_scale == 0.0f || (d.cell_scale >= 1.0f && std::isfinite(d.cell_scale))))
-        return fail(SPH_E_INVALID, "scene: cell_scale must be 0 or >= 1");
+    if (grid_reach(d.cell_scale) == 0)
+        return fail(SPH_E_INVALID, "scene: cell_scale must be 0, >= 1, or in [1/%d, 1)", kMaxReach);
     return SPH_OK;
 }
 
 // Grid of one scene from its box and particles: the finite walls, widened
 // to the particles; an open side gets head room of one particle extent (at
 // least 16 h).  Depends on nothing but the scene.
-void scene_grid(StepParams &P, const sph_scene_desc_v1 &d, const float *pos, uint32_t n)
+void scene_grid(BatchScene &b, const float *pos, uint32_t n)
 {
+    const sph_scene_desc_v1 &d = b.desc;
     double pmin[3] = {INFINITY, INFINITY, INFINITY}, pmax[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) {
@@ -57,8 +58,8 @@ void scene_grid(StepParams &P, const sph_scene_desc_v1 &d, const float *pos, uin
                 pmax[k] = std::max(pmax[k], v);
             }
         }
-    double lo[3], hi[3];
-    const double h = P.h;
+    double *lo = b.lo, *hi = b.hi;
+    const double h = b.P.h;
     for (int k = 0; k < 3; ++k) {
         if (!(pmin[k] <= pmax[k])) {
             pmin[k] = 0.0;
@@ -69,9 +70,15 @@ void scene_grid(StepParams &P, const sph_scene_desc_v1 &d, const float *pos, uin
         lo[k] = std::isfinite(blo) ? std::min(blo, pmin[k]) : pmin[k] - ext;
         hi[k] = std::isfinite(bhi) ? std::max(bhi, pmax[k]) : pmax[k] + ext;
     }
-    const double scale = d.cell_scale > 1.0f ? (double)d.cell_scale : 1.0;
-    const uint64_t max_cells = std::max<uint64_t>(64ull * n, 1ull << 16);
-    make_grid(P, lo, hi, scale, max_cells);
+    (void)scene_regrid(b, 0.0);  // desc.cell_scale was validated
+}
+
+int scene_regrid(BatchScene &b, double scale)
+{
+    if (!(scale > 0.0))
+        scale = b.desc.cell_scale > 0.0f ? (double)b.desc.cell_scale : 1.0;
+    const uint64_t max_cells = std::max<uint64_t>(64ull * b.desc.n, 1ull << 16);
+    return make_grid(b.P, b.lo, b.hi, scale, max_cells);
 }
 
 void Batch::drop_graph()
@@ -236,7 +243,7 @@ int Batch::upload(uint32_t s, const float *pos, const float *vel)
         return fail(SPH_E_INVALID, "upload: bad scene or NULL positions");
     BatchScene &b = scenes[s];
     const uint32_t n = b.desc.n, p0 = b.P.p_begin;
-    scene_grid(b.P, b.desc, pos, n);
+    scene_grid(b, pos, n);
     std::vector<float4> p4(n), v4(n);
     for (uint32_t i = 0; i < n; ++i) {
         p4[i] = make_float4(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2], 0.0f);
@@ -382,7 +389,7 @@ int Batch::step_timed(int nsteps, float *ms)
     return SPH_OK;
 }
 
-int Batch::autotune(int trial_steps, sph_tune_v1 *out)
+int Batch::autotune(int trial_steps, sph_tune_v1 *out, bool grid)
 {
     int rc;
     if (trial_steps <= 0 || !out)
@@ -411,38 +418,50 @@ int Batch::autotune(int trial_steps, sph_tune_v1 *out)
         keys_dirty = true;
         return SPH_OK;
     };
-    auto apply = [&](int block, int lane) {
+    // The walk (per lane or union) is not a knob: it may change split sums.
+    auto apply = [&](int block, double scale) -> int {
         w.stencil_block = block;
-        for (BatchScene &b : scenes)
-            b.P.lane_walk = (lane || (b.desc.flags & SPH_SCENE_LANE_WALK)) ? 1u : 0u;
-        layout_dirty = true;  // re-uploads the scene table
+        if (grid)
+            for (BatchScene &b : scenes) {
+                b.desc.cell_scale = (float)scale;  // kept by later uploads
+                int r = scene_regrid(b, 0.0);
+                if (r)
+                    return r;
+            }
+        layout_dirty = true;  // re-uploads the scene table (and re-keys on a new grid)
+        return SPH_OK;
     };
     const int blocks[3] = {64, 128, 256};
+    const double scales[2] = {1.0, 0.5};
+    const int nscales = grid ? 2 : 1;
     std::vector<float> ms((size_t)trial_steps);
-    double best = -1.0;
+    double best = -1.0, best_scale = 0.0;
     int k = 0;
-    for (int lane = 0; lane < 1; ++lane)  // the walk stays as planned: it may change split sums
+    for (int si = 0; si < nscales; ++si)
         for (int bi = 0; bi < 3; ++bi, ++k) {
-            apply(blocks[bi], lane);
-            if ((rc = restore()) || (rc = step_timed(1, nullptr)) || (rc = restore()) ||
-                (rc = step_timed(trial_steps, ms.data())))
+            const double scale = grid ? scales[si] : 0.0;
+            if ((rc = apply(blocks[bi], scale)) || (rc = restore()) || (rc = step_timed(1, nullptr)) ||
+                (rc = restore()) || (rc = step_timed(trial_steps, ms.data())))
                 return rc;
             double sum = 0.0;
             for (float v : ms)
                 sum += v;
             const double mean = sum / trial_steps;
             out->trial_ms[k] = mean;
-            if (blocks[bi] == kStencilBlock && lane == 0)
+            if (blocks[bi] == kStencilBlock && si == 0)
                 out->default_ms_per_step = mean;
             if (best < 0.0 || mean < best) {
                 best = mean;
+                best_scale = scale;
                 out->stencil_block = blocks[bi];
-                out->lane_walk = lane;
             }
         }
     out->ms_per_step = best;
     out->trials = k;
-    apply(out->stencil_block, out->lane_walk);
+    out->lane_walk = 0;
+    out->cell_scale = (float)best_scale;
+    if ((rc = apply(out->stencil_block, best_scale)))
+        return rc;
     if ((rc = restore()))
         return rc;
     e = hipStreamSynchronize(stream);
@@ -761,6 +780,20 @@ int sph_hip_autotune(sph_ctx *c, int trial_steps, sph_tune_v1 *out)
     if (!c)
         return fail(SPH_E_INVALID, "autotune: NULL context");
     return c->b.autotune(trial_steps, out);
+}
+
+int sph_batch_autotune_grid(sph_batch *b, int trial_steps, sph_tune_v1 *out)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "batch autotune: NULL batch");
+    return b->b.autotune(trial_steps, out, true);
+}
+
+int sph_hip_autotune_grid(sph_ctx *c, int trial_steps, sph_tune_v1 *out)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "autotune: NULL context");
+    return c->b.autotune(trial_steps, out, true);
 }
 
 }  // extern "C"

@@ -50,6 +50,7 @@ constexpr int kGridMaxY = 1024;
 constexpr int kGridMaxZ = 1024;
 constexpr uint32_t kSlowBit = 0x80000000u;
 constexpr double kCellMargin = 1.0 / 1024.0;  // H = h (1 + 2^-10)
+constexpr int kMaxReach = 3;  // finest grid: cell edge h (1 + 2^-10) / 3
 constexpr float kQMax = 8192.0f;              // |x/h| beyond this: always slow
 
 // Per-scene constants used by every pass.  Derived on the host from the
@@ -69,6 +70,9 @@ struct StepParams {
     double grid_o[3];
     double grid_inv;  // 1 / H
     int nx, ny, nz;
+    // stencil reach R in cells: H >= h (1 + 2^-10) / R, so a neighbour within
+    // h lies at most R cells away along each axis (1 <= R <= kMaxReach)
+    int reach;
     uint32_t cell_base;  // first linear cell of this scene
     // slots [p_begin, p_end) of the particle arrays belong to this scene
     uint32_t p_begin, p_end;

@@ -48,7 +48,6 @@ constexpr uint32_t kPartRoomMin = 49152;   // ... raised to this many per scene 
 constexpr uint32_t kPartRoomMaxPerWave = 16;  // ... but at most this many per wave task
 constexpr uint32_t kLaneWalkWeight = 3;    // per-lane walk candidates count this many times
 constexpr uint32_t kSlowWalkWeight = 8;    // ... in a wave with slow lanes (m_ij counted per neighbour)
-constexpr uint32_t kDensityPartGroup = 2;  // plan parts walked by one density part task (a power of 2)
 
 struct ParticleArrays {
     float4 *posA, *velA, *posB, *velB;

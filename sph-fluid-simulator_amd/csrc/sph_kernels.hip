@@ -504,28 +504,23 @@ __device__ inline RefCell ref_cell(float4 p, const StepParams &P)
     return RefCell{trunc_to_int(p.x / P.h), trunc_to_int(p.y / P.h), trunc_to_int(p.z / P.h)};
 }
 
-// The 9 rows of the 3x3x3 grid-cell stencil around packed cell cc: for row
-// (dy, dz) the x-run of three cells is one contiguous range [b, e) of the
-// cell-ordered arrays.  Calls row(b, e) for each row inside the grid.
+// The (2R + 1)^2 rows of the (2R + 1)^3 grid-cell stencil around packed
+// cell cc (R = P.reach), clipped to the grid: for row (y, z) the x-run of
+// cells [gx - R, gx + R] is one contiguous range [b, e) of the cell-ordered
+// arrays.  Calls row(b, e) for each row, z-major then y, both ascending.
 template <class Row>
 __device__ inline void for_each_row(const uint32_t *__restrict__ cstart, uint32_t cc,
                                     const StepParams &P, Row row)
 {
-    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc);
-    const int x0 = gx > 0 ? gx - 1 : 0;
-    const int x1 = gx + 1 < P.nx ? gx + 1 : P.nx - 1;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = gz + dz;
-        if (z < 0 || z >= P.nz)
-            continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = gy + dy;
-            if (y < 0 || y >= P.ny)
-                continue;
+    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc), R = P.reach;
+    const int x0 = max(gx - R, 0), x1 = min(gx + R, P.nx - 1);
+    const int y0 = max(gy - R, 0), y1 = min(gy + R, P.ny - 1);
+    const int z0 = max(gz - R, 0), z1 = min(gz + R, P.nz - 1);
+    for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y) {
             const uint32_t r0 = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
             row(cstart[r0 + (uint32_t)x0], cstart[r0 + (uint32_t)x1 + 1u]);
         }
-    }
 }
 
 // for_each_row restricted to the positions [lo, hi) of the lane's own
@@ -560,6 +555,10 @@ __device__ inline int wave_max_i(int x)
     return __builtin_amdgcn_readfirstlane(x);
 }
 
+// Most cells a union walk's wave may span in y and in z: the union's
+// (span + 2R)^2 rows fit the 64 lanes that stage them (R = 1: 4, 2: 4, 3: 2).
+__device__ inline int union_span(int R) { return min(4, 8 - 2 * R); }
+
 // A box of grid cells [x0, x1] x [y0, y1] x [z0, z1] (inside the grid)
 // whose rows a wave walks with one candidate index for all of its lanes.
 struct UnionRows {
@@ -572,19 +571,20 @@ __device__ inline uint32_t row_base(const StepParams &P, int y, int z)
 }
 
 // Chooses between the two ways a wave walks its candidates:
-//  * per lane: each lane walks its own 3x3x3 stencil (gathers);
+//  * per lane: each lane walks its own (2R + 1)^3 stencil (gathers);
 //  * union: all lanes walk the union of their stencils together, so the
 //    candidate index is wave-uniform and its record is a scalar load
 //    broadcast to every lane.
 // Union is taken when all lanes sit in one cell (the dense regime: one
-// cell holds many waves), or when their cells span at most 6 x 4 x 4 and
-// the union holds at most 3x the largest own stencil (a per-lane walk
-// costs about three times as much per candidate).  A candidate the
-// union adds to a lane's own stencil lies in a cell two or more cells away,
-// more than H >= h (1 + 2^-10) apart along some axis, so its float squared
-// distance is above h^2 and it never passes: both walks visit a lane's
-// passing candidates in the same order (rows z-major, then y, slots
-// ascending) and give bit-identical sums.
+// cell holds many waves), or when their cells span at most 6 cells in x
+// and union_span(R) in y and z (so the union has at most 64 rows) and the
+// union holds at most 3x the largest own stencil (a per-lane walk costs
+// about three times as much per candidate).  A candidate the union adds
+// to a lane's own stencil lies in a cell R + 1 or more cells away, beyond
+// R whole cells of edge >= h (1 + 2^-10) / R along some axis, so its
+// float squared distance is above h^2 and it never passes: both walks
+// visit a lane's passing candidates in the same order (rows z-major, then
+// y, slots ascending) and give bit-identical sums.
 __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t cc, const StepParams &P,
                                   UnionRows &U, uint32_t &count)
 {
@@ -606,15 +606,17 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
         y1 = wave_max_i(gy);
         z0 = wave_min_i(gz);
         z1 = wave_max_i(gz);
-        if (x1 - x0 > 5 || y1 - y0 > 3 || z1 - z0 > 3)
+        const int span = union_span(P.reach);
+        if (x1 - x0 > 5 || y1 - y0 >= span || z1 - z0 >= span)
             return false;
     }
-    U.x0 = x0 > 0 ? x0 - 1 : 0;
-    U.x1 = x1 + 1 < P.nx ? x1 + 1 : P.nx - 1;
-    U.y0 = y0 > 0 ? y0 - 1 : 0;
-    U.y1 = y1 + 1 < P.ny ? y1 + 1 : P.ny - 1;
-    U.z0 = z0 > 0 ? z0 - 1 : 0;
-    U.z1 = z1 + 1 < P.nz ? z1 + 1 : P.nz - 1;
+    const int R = P.reach;
+    U.x0 = max(x0 - R, 0);
+    U.x1 = min(x1 + R, P.nx - 1);
+    U.y0 = max(y0 - R, 0);
+    U.y1 = min(y1 + R, P.ny - 1);
+    U.z0 = max(z0 - R, 0);
+    U.z1 = min(z1 + R, P.nz - 1);
     uint32_t un = 0;
     for (int z = U.z0; z <= U.z1; ++z)
         for (int y = U.y0; y <= U.y1; ++y) {
@@ -651,7 +653,7 @@ __device__ inline bool walk_of(const PartPlan &pp, uint32_t wt, const uint32_t *
     return plan_union(cstart, cc, P, U, count) && !__any(slow);
 }
 
-// Length of the lane's own candidate sequence (its 3x3x3 stencil).
+// Length of the lane's own candidate sequence (its (2R + 1)^3 stencil).
 __device__ inline uint32_t own_length(const PartPlan &pp, uint32_t s, const uint32_t *__restrict__ cstart, uint32_t cc,
                                       const StepParams &P)
 {
@@ -714,7 +716,7 @@ __device__ inline void staged_union_walk(const uint32_t *__restrict__ cstart, co
 {
     const uint32_t lane = threadIdx.x & 63u;
     const int ny = U.y1 - U.y0 + 1;
-    const int nrows = ny * (U.z1 - U.z0 + 1);  // at most 6 x 6 <= 64 lanes
+    const int nrows = ny * (U.z1 - U.z0 + 1);  // at most (union_span + 2R)^2 <= 64 lanes
     uint32_t b = 0, len = 0;
     if ((int)lane < nrows) {
         const uint32_t r0 = row_base(P, U.y0 + (int)lane % ny, U.z0 + (int)lane / ny);
@@ -991,16 +993,6 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
     NeighborDump dump, PartPlan pp)
 {
     for_each_part<BLOCK>(S, pp, pp.queue, [&](uint32_t wt, uint32_t part, uint32_t nparts) {
-        // The density pass takes the plan's parts kDensityPartGroup at a time
-        // (its work per candidate is a fraction of the force pass's, so its
-        // tasks would be short next to their set-up): part p = g k walks the
-        // slice of parts [p, p + g) as density part k of ceil(nparts / g);
-        // the other parts are skipped.
-        constexpr uint32_t g = kDensityPartGroup;
-        if (part % g)
-            return;
-        const uint32_t part_end = min(part + g, nparts);
-        const uint32_t dpart = part / g, dparts = (nparts + g - 1u) / g;
         Lane L;
         bool active;
         wave_task(S, wt, L, active);
@@ -1038,14 +1030,14 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         };
         // Waves with a slow-multiplicity lane (rare where the union pays)
         // take the per-lane walk, so the union walk needs no m_ij.  A split
-        // task walks its slice [part, part_end) / nparts of the sequence.
+        // task walks its slice [part, part + 1) / nparts of the sequence.
         UnionRows U;
         uint32_t ucount = 0;
         if (MODE == 0 && walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {
             // Wave-uniform candidate j: scalar loads, four records at a time
             // (density is too short per candidate to feed from LDS).
             const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
-            const uint32_t hi = (uint32_t)((uint64_t)ucount * part_end / nparts);
+            const uint32_t hi = (uint32_t)((uint64_t)ucount * (part + 1u) / nparts);
             auto cand = [&](uint32_t j, float4 q, auto self_tag) {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
@@ -1073,10 +1065,10 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             });
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
-            if (dparts > 1u) {
+            if (nparts > 1u) {
                 const uint32_t own = own_length(pp, s, cstart, cc, P);
                 lo = (uint32_t)((uint64_t)own * part / nparts);
-                hi = (uint32_t)((uint64_t)own * part_end / nparts);
+                hi = (uint32_t)((uint64_t)own * (part + 1u) / nparts);
             }
             for_each_row_span(cstart, cc, P, lo, hi, [&](uint32_t b, uint32_t e) {
                 uint32_t j = b;
@@ -1092,9 +1084,9 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                     body(j, pos[j]);
             });
         }
-        if (dparts > 1u) {
+        if (nparts > 1u) {
             float v[1] = {acc};
-            if (!combine_parts<1>(pp, wt, dpart, dparts, v))
+            if (!combine_parts<1>(pp, wt, part, nparts, v))
                 return;
             acc = v[0];
         }

@@ -24,6 +24,10 @@ int check_settings(const sph_settings_v1 *s);
 StepParams make_params(const sph_settings_v1 &s, const sph_box_v1 &box, float dt);
 int make_grid(StepParams &P, const double lo[3], const double hi[3], double cell_scale,
               uint64_t max_cells);
+// Stencil reach in cells of a grid cell scale (cell edge in units of
+// h (1 + 2^-10)): 1 for 0 (the default) and scales >= 1, ceil(1 / scale)
+// for scales in [1 / kMaxReach, 1), 0 (invalid) otherwise.
+int grid_reach(double cell_scale);
 
 // Growable device allocation (hipMalloc on growth only).
 struct DevBuf {

@@ -125,10 +125,23 @@ int check_settings(const sph_settings_v1 *s)
     return SPH_OK;
 }
 
+int grid_reach(double cell_scale)
+{
+    if (cell_scale == 0.0 || cell_scale >= 1.0)
+        return std::isfinite(cell_scale) ? 1 : 0;
+    if (!(cell_scale * kMaxReach >= 1.0))
+        return 0;
+    int R = (int)std::ceil(1.0 / cell_scale);
+    if (cell_scale * R < 1.0)  // 1 / scale rounded down
+        ++R;
+    return R <= kMaxReach ? R : 0;
+}
+
 StepParams make_params(const sph_settings_v1 &s, const sph_box_v1 &box, float dt)
 {
     StepParams P;
     std::memset(&P, 0, sizeof P);
+    P.reach = 1;
     P.h = s.h;
     P.h2 = s.h2;
     P.mass = s.mass;
@@ -157,10 +170,18 @@ StepParams make_params(const sph_settings_v1 &s, const sph_box_v1 &box, float dt
 // one padding cell on each side, per-axis dims within the packing limits
 // and at most max_cells cells in total (H grows until both hold; larger
 // cells only add candidates).  Positions outside clamp to the border cells.
+// A scale below 1 (at least 1 / kMaxReach) makes cells finer than h: the
+// stencil then reaches R = grid_reach(scale) cells, and H * R >= h (1 + 2^-10).
 int make_grid(StepParams &P, const double lo[3], const double hi[3], double cell_scale,
               uint64_t max_cells)
 {
-    double H = (double)P.h * (1.0 + kCellMargin) * std::max(1.0, cell_scale);
+    const int R = grid_reach(cell_scale);
+    if (R == 0)
+        return fail(SPH_E_INVALID, "cell scale %g is not 0, >= 1, or in [1/%d, 1)", cell_scale, kMaxReach);
+    if (cell_scale == 0.0)
+        cell_scale = 1.0;
+    P.reach = R;
+    double H = (double)P.h * (1.0 + kCellMargin) * cell_scale;
     const int lim[3] = {kGridMaxX, kGridMaxY, kGridMaxZ};
     int dims[3];
     for (int iter = 0; iter < 200; ++iter) {
@@ -764,6 +785,7 @@ int sph_classify_host(const float pos[3], float h, int32_t out[5])
     std::memset(&P, 0, sizeof P);
     P.h = h;
     P.nx = P.ny = P.nz = 1;
+    P.reach = 1;
     P.grid_inv = 1.0;
     Classified c = classify(pos[0], pos[1], pos[2], P);
     float qx = pos[0] / h, qy = pos[1] / h, qz = pos[2] / h;

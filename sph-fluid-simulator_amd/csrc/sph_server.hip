@@ -112,7 +112,7 @@ int Server::submit(const sph_scene_desc_v1 &d, const float *pos, const float *ve
     } catch (...) {
         return fail(SPH_E_NOMEM, "submit: host allocation failed");
     }
-    scene_grid(r->scene.P, d, r->pos.data(), d.n);
+    scene_grid(r->scene, r->pos.data(), d.n);
     r->bytes = bytes_for(r->scene);
     if (r->bytes > cfg.memory_budget)
         return fail(SPH_E_LIMIT, "submit: scene needs %llu bytes, budget is %llu",

@@ -74,12 +74,12 @@ class SceneDesc(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("stencil_block", ctypes.c_int32), ("lane_walk", ctypes.c_int32), ("trials", ctypes.c_int32),
-                ("_pad", ctypes.c_int32), ("ms_per_step", c_double), ("default_ms_per_step", c_double),
+                ("cell_scale", ctypes.c_float), ("ms_per_step", c_double), ("default_ms_per_step", c_double),
                 ("trial_ms", c_double * 6)]
 
     def as_dict(self):
         return {"stencil_block": self.stencil_block, "lane_walk": self.lane_walk, "trials": self.trials,
-                "ms_per_step": self.ms_per_step, "default_ms_per_step": self.default_ms_per_step,
+                "cell_scale": self.cell_scale, "ms_per_step": self.ms_per_step, "default_ms_per_step": self.default_ms_per_step,
                 "trial_ms": list(self.trial_ms)}
 
 
@@ -154,6 +154,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_batch_step_timed": (c_int, [c_void_p, c_int, c_void_p]),
         "sph_batch_autotune": (c_int, [c_void_p, c_int, POINTER(Tune)]),
         "sph_hip_autotune": (c_int, [c_void_p, c_int, POINTER(Tune)]),
+        "sph_batch_autotune_grid": (c_int, [c_void_p, c_int, POINTER(Tune)]),
+        "sph_hip_autotune_grid": (c_int, [c_void_p, c_int, POINTER(Tune)]),
         "sph_server_create": (c_int, [POINTER(ServerConfig), POINTER(c_void_p)]),
         "sph_server_destroy": (c_int, [c_void_p]),
         "sph_server_submit": (c_int, [c_void_p, POINTER(SceneDesc), c_void_p, c_void_p, c_uint32,
@@ -329,9 +331,12 @@ class Context:
         _check(self._lib.sph_hip_download_particles(self._h, _ptr(p), _ptr(xf)))
         return p, xf
 
-    def autotune(self, trial_steps: int = 10) -> dict:
+    def autotune(self, trial_steps: int = 10, grid: bool = False) -> dict:
+        """Picks the stencil block size (bit-preserving); with grid=True also
+        the cell scale, 1 or 1/2 (changes results by rounding)."""
         t = Tune()
-        _check(self._lib.sph_hip_autotune(self._h, trial_steps, ctypes.byref(t)))
+        fn = self._lib.sph_hip_autotune_grid if grid else self._lib.sph_hip_autotune
+        _check(fn(self._h, trial_steps, ctypes.byref(t)))
         return t.as_dict()
 
     def profile(self, nsteps: int = 10) -> dict:
@@ -414,9 +419,11 @@ class Batch:
         _check(self._lib.sph_batch_step_timed(self._h, nsteps, _ptr(ms)))
         return ms
 
-    def autotune(self, trial_steps: int = 10) -> dict:
+    def autotune(self, trial_steps: int = 10, grid: bool = False) -> dict:
+        """As Context.autotune, for every scene of the batch together."""
         t = Tune()
-        _check(self._lib.sph_batch_autotune(self._h, trial_steps, ctypes.byref(t)))
+        fn = self._lib.sph_batch_autotune_grid if grid else self._lib.sph_batch_autotune
+        _check(fn(self._h, trial_steps, ctypes.byref(t)))
         return t.as_dict()
 
     def profile(self, nsteps: int = 10) -> dict:

@@ -15,6 +15,8 @@ from __future__ import annotations
 
 import numpy as np
 
+import sph_scenes
+
 F32 = np.float32
 RTOL = 1e-5
 
@@ -98,3 +100,19 @@ def assert_force(f, f_ref, scale, label=""):
     assert not bad.any(), f"{label} force: {bad.sum()} outside scaled tolerance, worst ratio " \
                           f"{np.max(df / np.maximum(tol, 1e-30)):.3e}"
     return float(np.max(df / np.maximum(tol, 1e-30))) if len(df) else 0.0
+
+
+def collision_clump(n=600, h=0.05, seed=9):
+    """A clump of n particles (side 0.4 h) around (0.3 h, h, -h): it spans
+    reference cells (0,0,0), (0,1,-1), (0,1,0) and (0,0,-1).  The buckets of
+    (0,-1,1) and (0,1,-1) coincide, so particles of cell (0,0,0) are slow
+    and count their neighbours in (0,1,-1) twice.  Every lane's stencil
+    holds the whole clump, so waves take the union walk, and n is small
+    enough that no wave task is split."""
+    rng = np.random.default_rng(seed)
+    centre = np.array([0.3 * h, h, -h])
+    pos = (centre + rng.uniform(-0.2 * h, 0.2 * h, size=(n, 3))).astype(F32)
+    vel = rng.normal(0.0, 0.05, size=(n, 3)).astype(F32)
+    mass = float(F32(0.02 * (h / 0.15) ** 3))
+    return sph_scenes.Scene("collision-clump", pos, vel, mass, 1000.0, 1.0, 1.04, h, -9.8, 0.2,
+                            0.001, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), steps=10)

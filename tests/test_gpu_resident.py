@@ -237,28 +237,12 @@ def test_split_walks_deterministic_and_agree(sph, orc32):
     assert np.all(dv <= 1e-3 * dv_ref + 1e-5), float(np.max(dv - 1e-3 * dv_ref))
 
 
-def _collision_clump(n=600, h=0.05, seed=9):
-    """A clump of n particles (side 0.4 h) around (0.3 h, h, -h): it spans
-    reference cells (0,0,0), (0,1,-1), (0,1,0) and (0,0,-1).  The buckets of
-    (0,-1,1) and (0,1,-1) coincide, so particles of cell (0,0,0) are slow
-    and count their neighbours in (0,1,-1) twice.  Every lane's stencil
-    holds the whole clump, so waves take the union walk, and n is small
-    enough that no wave task is split."""
-    rng = np.random.default_rng(seed)
-    centre = np.array([0.3 * h, h, -h])
-    pos = (centre + rng.uniform(-0.2 * h, 0.2 * h, size=(n, 3))).astype(F32)
-    vel = rng.normal(0.0, 0.05, size=(n, 3)).astype(F32)
-    mass = float(F32(0.02 * (h / 0.15) ** 3))
-    return sph_scenes.Scene("collision-clump", pos, vel, mass, 1000.0, 1.0, 1.04, h, -9.8, 0.2,
-                            0.001, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), steps=10)
-
-
 def test_union_walk_with_slow_lanes(sph, orc32):
     """Waves holding slow lanes (colliding buckets: m_ij = 2) take the union
     walk: one step matches the oracle (density, pressure, force), and the
     union and per-lane walks stay bit-identical over several steps."""
     import brute
-    sc = _collision_clump()
+    sc = pu.collision_clump()
     cells = brute.ref_cells(sc.pos, sc.h)
     slow = np.array([brute.buckets_collide_brute(c) for c in cells])
     assert slow.sum() >= 64, "premise: at least a wave's worth of slow particles"

@@ -165,16 +165,17 @@ def test_fast_path_theorem_boundary_adversarial(sph):
     _fast_path_holds(sph, pos, float(h))
 
 
-def test_grid_cells_of_neighbours_are_adjacent():
-    """Completeness of the 3x3x3 grid stencil: with H = h (1 + 2^-10) and
-    double-precision grid coordinates, any pair with fl(d^2) < fl(h*h) lies
-    in adjacent grid cells, including pairs at distance h - ulp across a
-    grid boundary."""
+@pytest.mark.parametrize("reach", [1, 2, 3])
+def test_grid_cells_of_neighbours_are_adjacent(reach):
+    """Completeness of the (2R + 1)^3 grid stencil: with cell edge
+    H = h (1 + 2^-10) / R and double-precision grid coordinates, any pair
+    with fl(d^2) < fl(h*h) lies at most R cells apart per axis, including
+    pairs at distance h - ulp across a grid boundary."""
     h = F32(0.03)
-    H = float(h) * (1.0 + 2.0 ** -10)
+    H = float(h) * (1.0 + 2.0 ** -10) / reach
     origin = np.array([-2.0 - H, -1.0 - H, -1.0 - H])
     inv = 1.0 / H
-    dims = (140, 70, 70)
+    dims = (140 * reach, 70 * reach, 70 * reach)
     rng = np.random.default_rng(5)
     pairs = []
     for _ in range(20000):
@@ -193,7 +194,7 @@ def test_grid_cells_of_neighbours_are_adjacent():
         d2 = (dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2]
         if d2 < h2:
             checked += 1
-            assert np.all(np.abs(gc[i] - gc[j]) <= 1)
+            assert np.all(np.abs(gc[i] - gc[j]) <= reach)
     assert checked > 1000
 
 
@@ -213,3 +214,14 @@ def test_invalid_arguments_fail_without_touching_a_device(sph):
     h = ctypes.c_void_p()
     assert lib.sph_hip_create(ctypes.byref(d), ctypes.byref(h)) == sph.SPH_E_INVALID
     assert lib.sph_hip_step(None, 1) == sph.SPH_E_INVALID
+
+
+@pytest.mark.parametrize("scale", [0.3, 0.25, -1.0, float("nan"), float("inf")])
+def test_invalid_cell_scale_rejected(sph, scale):
+    """Cell scales other than 0, >= 1 or [1/3, 1) are rejected before any
+    device work."""
+    sc = sph.settings(h=0.15)
+    d = sph.scene_desc(5, sc, sph.box((-1, -1, -1), (1, 1, 1)), 0.003, cell_scale=scale)
+    h = ctypes.c_void_p()
+    assert sph.load().sph_hip_create(ctypes.byref(d), ctypes.byref(h)) == sph.SPH_E_INVALID
+    assert b"cell_scale" in sph.load().sph_hip_last_error()
