@@ -94,13 +94,15 @@ SPH_HD inline uint64_t part_room(uint64_t waves)
 // sequence, with P = ceil(length / part size) up to kMaxParts; the part size
 // is kPartCandidates, raised when the scene's total walk over part_room
 // would exceed it (a scene's split depends on the scene alone, so it is the
-// same in any batch).  Parts are separate tasks pulled from a queue; the
-// last one to finish adds the partial sums in part order, so results are
+// same in any batch).  Unsplit tasks are dealt to the waves statically;
+// parts are separate tasks pulled from a queue afterwards; the last one to
+// finish adds the partial sums in part order, so results are
 // deterministic.  An unsplit task sums in candidate order (and its two
 // walks agree bit for bit); a split one differs from it by rounding only.
 struct PartPlan {
-    uint32_t *nparts;    // per wave task, padded to a kScanChunk multiple (zero outside use)
-    uint32_t *off;       // exclusive scan of nparts; off[nwaves] = number of part tasks
+    uint32_t *nparts;    // per wave task: parts of a split task, 0 for an unsplit one; padded
+                         // to a kScanChunk multiple (zero outside use)
+    uint32_t *off;       // exclusive scan of nparts; off[nwaves] = number of queued part tasks
     uint32_t *scan_partials;
     uint32_t padded;     // padded length of nparts
     uint32_t *measure;   // walk length per wave task

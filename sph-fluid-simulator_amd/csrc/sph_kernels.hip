@@ -864,7 +864,8 @@ __global__ void k_count_parts(const uint32_t *__restrict__ measure, SceneSet S,
     const unsigned long long room = part_room(waves);
     const unsigned long long size = max((unsigned long long)kPartCandidates, (total[scene] + room - 1u) / room);
     const unsigned long long p = (measure[g] + size - 1u) / size;
-    nparts[g] = (uint32_t)min(max(p, 1ull), (unsigned long long)kMaxParts);
+    const uint32_t np = (uint32_t)min(p, (unsigned long long)kMaxParts);
+    nparts[g] = np > 1u ? np : 0u;  // an unsplit task is not queued
 }
 
 __global__ void k_emit_parts(const uint32_t *__restrict__ off, uint32_t ntasks, uint32_t *__restrict__ tasks,
@@ -906,28 +907,28 @@ void launch_plan_parts(hipStream_t st, const ParticleArrays &a, const CellArrays
 
 // Runs task(w, part, nparts) for the work the calling wave takes.  Without
 // a plan each wave runs wave task w of its own index, unsplit.  With one,
-// waves pull part tasks from the queue counter until the list is exhausted
-// (every wave exits then).  All 64 lanes must be live on entry.
+// each wave first runs its static, XCD-contiguous share of the unsplit
+// tasks -- wave v takes tasks v, v + G, ... that are not split, with no
+// queue traffic, neighbouring tasks sharing an L2 -- then pulls the parts
+// of split tasks from the queue counter until that list is exhausted
+// (every wave exits then), which also evens out the static share.  All 64
+// lanes must be live on entry.
 template <int BLOCK, class Task>
 __device__ inline void for_each_part(const SceneSet &S, const PartPlan &pp, uint32_t *queue, Task task)
 {
+    const uint32_t first = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
     if (!queue) {
-        const uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
-        if (w < S.nwaves)
-            task(w, 0u, 1u);
+        if (first < S.nwaves)
+            task(first, 0u, 1u);
         return;
     }
-    const uint32_t total = __builtin_amdgcn_readfirstlane(pp.off[S.nwaves]);
-    if (total == S.nwaves) {
-        // No task is split this step (uniform, short walks): a static,
-        // XCD-contiguous assignment -- wave v takes tasks v, v + G, ... --
-        // with no queue traffic, neighbouring tasks sharing an L2.
-        const uint32_t G = gridDim.x * (BLOCK / 64);
-        for (uint32_t w = __builtin_amdgcn_readfirstlane(xcd_block() * (BLOCK / 64) + (threadIdx.x >> 6));
-             w < S.nwaves; w += G)
+    const uint32_t total = __builtin_amdgcn_readfirstlane(pp.off[S.nwaves]);  // parts of split tasks
+    const uint32_t G = gridDim.x * (BLOCK / 64);
+    for (uint32_t w = first; w < S.nwaves; w += G)
+        if (total == 0u || __builtin_amdgcn_readfirstlane(pp.off[w + 1] - pp.off[w]) == 0u)
             task(w, 0u, 1u);
+    if (total == 0u)
         return;
-    }
     for (;;) {
         uint32_t t = 0;
         if ((threadIdx.x & 63u) == 0u)
