@@ -22,6 +22,9 @@ def _mixed_scenes():
     scenes.append(app)
     w0 = sph_scenes.workload(0)  # open ceiling, different h and dt
     scenes.append(w0)
+    # clumps: its long walks split into parts (queued) while the other
+    # scenes' short tasks run in the waves' static share
+    scenes.append(sph_scenes.blobs(nblobs=4, per_blob=3000))
     return scenes
 
 
@@ -45,6 +48,18 @@ def test_batch_scene_equals_scene_alone(sph):
             ctx.step(steps)
             alone = _state(ctx.download())
         assert alone == mixed[k], f"scene {k} ({sc.name}) differs alone vs batched"
+    # premise: the clumped scene has split tasks (only a split task makes the
+    # union and per-lane walks differ in the last bits)
+    sc = scenes[-1]
+    walks = []
+    for lane_walk in (False, True):
+        with sph.Context(sc.n, sph.settings(*sc.settings_args()),
+                         sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt,
+                         lane_walk=lane_walk) as ctx:
+            ctx.upload(sc.pos, sc.vel)
+            ctx.step(1)
+            walks.append(_state(ctx.download()))
+    assert walks[0] != walks[1], "premise: the blob scene splits some wave tasks"
 
 
 def test_batch_order_of_scenes_does_not_matter(sph):
