@@ -46,7 +46,7 @@ constexpr uint32_t kPartRoomPerWave = 3;   // spare part tasks per wave task of 
 constexpr uint32_t kPartRoomMin = 49152;   // ... raised to this many per scene (~10 rounds of the
                                            // resident waves, so tasks stay short in mid-size scenes)
 constexpr uint32_t kPartRoomMaxPerWave = 16;  // ... but at most this many per wave task
-constexpr uint32_t kLaneWalkWeight = 3;    // per-lane walk candidates count this many times
+constexpr uint32_t kLaneWalkWeight = 5;    // per-lane walk candidates count this many times
 constexpr uint32_t kSlowWalkWeight = 8;    // ... in a wave with slow lanes (m_ij counted per neighbour)
 
 struct ParticleArrays {
