@@ -380,13 +380,101 @@ void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c
                        a.cellcA, a.rankA, c.cstart, a.perm, S);
 }
 
-// Big cells: one block sorts the A indices of one listed cell in LDS
-// (bitonic, ascending), so slot order inside the cell is the stable order.
-// Blocks walk the list with a stride; every block exits when the list ends.
+// Big cells: one block sorts the A indices of one listed cell (bitonic,
+// ascending), so slot order inside the cell is the stable order.  Thread t
+// holds elements t + kBigSortBlock j in registers: compare-exchanges at
+// strides >= kBigSortBlock stay in the thread, strides < 64 are wave
+// shuffles, and only strides 64 .. kBigSortBlock / 2 go through LDS (22 of
+// the 91 stages of an 8192-element sort).  Blocks walk the list with a
+// stride; every block exits when the list ends.
+constexpr int kBigSortRegs = (int)(kBigCellSortMax / kBigSortBlock);
+
+// Stages at stride Q kBigSortBlock: element j pairs with j | Q in the same
+// thread (Q and the element count RR compile-time, so the register indices
+// are fixed).
+template <int RR, uint32_t Q>
+__device__ inline void sort_stage_in_thread(uint32_t (&x)[RR], uint32_t t, uint32_t size)
+{
+#pragma unroll
+    for (int j = 0; j < RR; ++j) {
+        if (((uint32_t)j & Q) == 0u && (j | (int)Q) < RR) {
+            const int jj = j | (int)Q;
+            const bool up = ((t + (uint32_t)kBigSortBlock * (uint32_t)j) & size) == 0u;
+            const uint32_t lo = min(x[j], x[jj]), hi = max(x[j], x[jj]);
+            x[j] = up ? lo : hi;
+            x[jj] = up ? hi : lo;
+        }
+    }
+}
+
+// Sorts the len (<= RR kBigSortBlock, > 64) entries of perm at b; m is the
+// power of two >= len.  Called by every thread of the block.
+template <int RR>
+__device__ inline void sort_cell(uint32_t *__restrict__ perm, uint32_t b, uint32_t len, uint32_t m,
+                                 uint32_t *__restrict__ sh)
+{
+    constexpr uint32_t T = kBigSortBlock;
+    const uint32_t t = threadIdx.x;
+    // m >= 128 is a multiple of 64: a wave is wholly live or wholly idle
+    const bool live = t < m;
+    uint32_t x[RR];
+#pragma unroll
+    for (int j = 0; j < RR; ++j) {
+        const uint32_t k = t + T * (uint32_t)j;
+        x[j] = k < len ? perm[b + k] : 0xFFFFFFFFu;
+    }
+    for (uint32_t size = 2; size <= m; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride >= T) {
+                switch (stride / T) {
+                case 1: sort_stage_in_thread<RR, 1>(x, t, size); break;
+                case 2: sort_stage_in_thread<RR, 2>(x, t, size); break;
+                case 4: sort_stage_in_thread<RR, 4>(x, t, size); break;
+                default: sort_stage_in_thread<RR, 8>(x, t, size); break;
+                }
+            } else if (stride < 64u) {
+                if (live) {
+#pragma unroll
+                    for (int j = 0; j < RR; ++j) {
+                        const uint32_t k = t + T * (uint32_t)j;
+                        const uint32_t y = (uint32_t)__shfl_xor((int)x[j], (int)stride, 64);
+                        const bool up = (k & size) == 0u, low = (k & stride) == 0u;
+                        x[j] = (low == up) ? min(x[j], y) : max(x[j], y);
+                    }
+                }
+            } else {
+                if (live) {
+#pragma unroll
+                    for (int j = 0; j < RR; ++j)
+                        sh[t + T * (uint32_t)j] = x[j];
+                }
+                __syncthreads();
+                if (live) {
+#pragma unroll
+                    for (int j = 0; j < RR; ++j) {
+                        const uint32_t k = t + T * (uint32_t)j;
+                        const uint32_t y = sh[k ^ stride];
+                        const bool up = (k & size) == 0u, low = (k & stride) == 0u;
+                        x[j] = (low == up) ? min(x[j], y) : max(x[j], y);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RR; ++j) {
+        const uint32_t k = t + T * (uint32_t)j;
+        if (k < len)
+            perm[b + k] = x[j];
+    }
+}
+
 __global__ __launch_bounds__(kBigSortBlock) void k_sort_big_cells(
     uint32_t *__restrict__ perm, const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ big,
     uint32_t big_cap)
 {
+    static_assert(kBigSortRegs == 16 && kSmallCell >= 64, "sort layout");
     __shared__ uint32_t sh[kBigCellSortMax];
     const uint32_t nbig = min(big[0], big_cap);
     for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
@@ -397,28 +485,13 @@ __global__ __launch_bounds__(kBigSortBlock) void k_sort_big_cells(
         uint32_t m = 1;
         while (m < len)
             m <<= 1;
-        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x)
-            sh[k] = k < len ? perm[b + k] : 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t size = 2; size <= m; size <<= 1) {
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-                    const uint32_t partner = k ^ stride;
-                    if (partner > k) {
-                        const bool up = (k & size) == 0u;
-                        const uint32_t x = sh[k], y = sh[partner];
-                        if ((x > y) == up) {
-                            sh[k] = y;
-                            sh[partner] = x;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+        switch (m > (uint32_t)kBigSortBlock ? m / kBigSortBlock : 1u) {  // elements per thread
+        case 1: sort_cell<1>(perm, b, len, m, sh); break;
+        case 2: sort_cell<2>(perm, b, len, m, sh); break;
+        case 4: sort_cell<4>(perm, b, len, m, sh); break;
+        case 8: sort_cell<8>(perm, b, len, m, sh); break;
+        default: sort_cell<16>(perm, b, len, m, sh); break;
         }
-        for (uint32_t k = threadIdx.x; k < len; k += blockDim.x)
-            perm[b + k] = sh[k];
-        __syncthreads();
     }
 }
 
