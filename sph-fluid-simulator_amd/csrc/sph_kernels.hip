@@ -649,7 +649,7 @@ __device__ inline uint32_t row_base(const StepParams &P, int y, int z)
 //    candidate index is wave-uniform and its record is a scalar load
 //    broadcast to every lane.
 // Union is taken when all lanes sit in one cell (the dense regime: one
-// cell holds many waves), or when their cells span at most 6 cells in x
+// cell holds many waves), or when their cells span at most 16 cells in x
 // and union_span(R) in y and z (so the union has at most 64 rows) and the
 // union holds at most 3x the largest own stencil (a per-lane walk costs
 // about three times as much per candidate).  A candidate the union adds
@@ -680,7 +680,7 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
         z0 = wave_min_i(gz);
         z1 = wave_max_i(gz);
         const int span = union_span(P.reach);
-        if (x1 - x0 > 5 || y1 - y0 >= span || z1 - z0 >= span)
+        if (x1 - x0 > 15 || y1 - y0 >= span || z1 - z0 >= span)
             return false;
     }
     const int R = P.reach;
