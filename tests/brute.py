@@ -89,3 +89,29 @@ def reference_constants(mass, h):
     spiky_grad = -45.0 / (pi * h ** 6)
     spiky_lap = 45.0 / (pi * h ** 6)
     return {"poly6": poly6, "spikyGrad": spiky_grad, "spikyLap": spiky_lap, "selfDens": mass * poly6 * h ** 6}
+
+
+def forces_brute(pos, vel, rho, prs, rows, h, mass, visc, spiky_grad, spiky_lap, rho_of="j"):
+    """Reference force (src/sph.cpp:107-121) in double from the definition:
+    F_i = sum over the multiplicity-repeated row of i of
+        -d m (p_i + p_j) / (2 rho_j) spikyGrad (h - r)^2
+        + visc m (v_j - v_i) / rho_j spikyLap (h - r),
+    d = (x_j - x_i) / r.  rho_of="i" divides by rho_i instead (a planted
+    slip the pin must reject)."""
+    pos = np.asarray(pos, np.float64)
+    vel = np.asarray(vel, np.float64)
+    rho = np.asarray(rho, np.float64)
+    prs = np.asarray(prs, np.float64)
+    out = np.zeros((len(pos), 3))
+    for i, row in enumerate(rows):
+        if len(row) == 0:
+            continue
+        j = np.asarray(row, np.int64)
+        d = pos[j] - pos[i]
+        r = np.sqrt(np.sum(d * d, axis=1))
+        dirn = d / r[:, None]
+        rd = rho[j] if rho_of == "j" else np.full(len(j), rho[i])
+        fp = -dirn * (mass * (prs[i] + prs[j]) / (2.0 * rd) * spiky_grad * (h - r) ** 2)[:, None]
+        fv = visc * mass * (vel[j] - vel[i]) / rd[:, None] * (spiky_lap * (h - r))[:, None]
+        out[i] = np.sum(fp + fv, axis=0)
+    return out

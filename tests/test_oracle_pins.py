@@ -209,6 +209,36 @@ def test_two_particle_force_closed_form(orc32):
     assert fp < 0  # the reference's sign: negative pressure pushes the pair apart
 
 
+def test_force_matches_brute_force_on_asymmetric_cloud(orc32):
+    """Unequal densities and pressures (an axis cloud: rho spans ~15-100)
+    with m_ij = 2 neighbours: the oracle's force agrees with the double
+    brute force of src/sph.cpp:107-121 within the scaled parity bar, while
+    dividing by rho_i instead of rho_j, or dropping the multiplicity, misses
+    it by orders of magnitude."""
+    import parity_util as pu
+    h = 0.15
+    pos = _axis_cloud(600, h, 21)
+    rng = np.random.default_rng(22)
+    vel = rng.normal(0.0, 0.3, size=pos.shape).astype(F32)
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    box = orc32.box((-50, -50, -50), (50, 50, 50))
+    out, _ = _step_by_id(orc32, pos, vel, s, 0.0, box)
+    rho, prs = out["density"].astype(np.float64), out["pressure"].astype(np.float64)
+    assert rho.max() > 3.0 * rho.min()  # densities really differ
+    rows = brute.neighbors_brute(pos, h)
+    assert any(len(r) != len(np.unique(r)) for r in rows)  # m_ij = 2 occurs
+    args = (float(s.h), float(s.mass), float(s.viscosity), float(s.spikyGrad), float(s.spikyLap))
+    f_true = brute.forces_brute(pos, vel, rho, prs, rows, *args)
+    scale = pu.force_scale(pos, vel, rho, prs, rows, s)
+    worst = pu.assert_force(out["force"], f_true, scale, "oracle vs brute")
+    assert worst < 0.1
+    df = np.linalg.norm(out["force"] - brute.forces_brute(pos, vel, rho, prs, rows, *args, rho_of="i"), axis=1)
+    assert np.max(df / np.maximum(pu.RTOL * scale, 1e-30)) > 1e3
+    single = [np.unique(r) for r in rows]
+    df = np.linalg.norm(out["force"] - brute.forces_brute(pos, vel, rho, prs, single, *args), axis=1)
+    assert np.max(df / np.maximum(pu.RTOL * scale, 1e-30)) > 1e3
+
+
 def test_isolated_particle_has_zero_force_and_falls(orc32):
     s = orc32.settings(0.02, 1000.0, 1.0, 1.04, 0.15, -9.8, 0.2)
     box = orc32.box((-50, -50, -50), (50, 50, 50))
