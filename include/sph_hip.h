@@ -287,6 +287,14 @@ int sph_server_status(const sph_server *s, uint64_t request_id, int *state,
 int sph_server_fetch(sph_server *s, uint64_t request_id, float *pos,
                      float *vel, float *rho, float *pressure, uint32_t *ids);
 int sph_server_stats(const sph_server *s, sph_server_stats_v1 *out);
+/* Tests only: the next admission's relayout (SPH_SERVER_FAIL_RELAYOUT) or
+ * the next scene upload (SPH_SERVER_FAIL_UPLOAD) fails once, as a failed
+ * device allocation would.  The failing tick returns the error; its joining
+ * requests go back to the head of the queue and the running scenes keep
+ * their state, so the next tick admits them again. */
+#define SPH_SERVER_FAIL_RELAYOUT 1
+#define SPH_SERVER_FAIL_UPLOAD 2
+int sph_server_inject_failure(sph_server *s, int stage);
 /* GPU time of every batch step so far, oldest first (up to cap values). */
 int sph_server_step_times(const sph_server *s, float *ms, size_t cap,
                           size_t *n);

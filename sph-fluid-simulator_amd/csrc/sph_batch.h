@@ -39,6 +39,10 @@ struct Batch {
     hipGraphExec_t exec = nullptr;
     DevBuf aos, xf;  // download staging
     std::vector<hipEvent_t> events;
+    // Tests only: a failure the next relayout (kFailRelayout) or upload
+    // (kFailUpload) reports at its last fallible point, once.
+    static constexpr int kFailRelayout = 1, kFailUpload = 2;
+    int inject_failure = 0;
 
     Batch() = default;
     Batch(const Batch &) = delete;
@@ -53,6 +57,7 @@ struct Batch {
     // order, so it steps on bit-identically) or, for from[k] < 0, is a new
     // scene to upload before stepping.  Densities, pressures and forces of
     // continued scenes are not carried over (the next step recomputes them).
+    // On failure the batch keeps its old scenes and their state.
     int relayout(std::vector<BatchScene> next, const std::vector<int> &from);
     int upload(uint32_t s, const float *pos, const float *vel);
     // Lays out cells and the scene table and computes keys when needed.

@@ -217,15 +217,25 @@ int Batch::relayout(std::vector<BatchScene> next, const std::vector<int> &from)
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess)
         return hip_fail(e, "relayout copy");
+    if (inject_failure == kFailRelayout) {
+        inject_failure = 0;
+        return fail(SPH_E_NOMEM, "relayout: injected failure");
+    }
     if (grow) {
+        // np / nv keep the old state until every other array has grown, so
+        // a failure here leaves the batch on its old layout (a failed
+        // growth keeps each array as it was)
         swap_buf(w.posA, np);
         swap_buf(w.velA, nv);
-        np.release();
-        nv.release();
         const uint32_t want = (uint32_t)(w.posA.bytes / sizeof(float4));
         if ((rc = w.reserve_particles(want)) ||
-            (keep_force && (rc = w.force.reserve(sizeof(float4) * (size_t)want))))
+            (keep_force && (rc = w.force.reserve(sizeof(float4) * (size_t)want)))) {
+            swap_buf(w.posA, np);
+            swap_buf(w.velA, nv);
             return rc;
+        }
+        np.release();
+        nv.release();
     } else {
         swap_buf(w.posA, w.posB);
         swap_buf(w.velA, w.velB);
@@ -241,6 +251,10 @@ int Batch::upload(uint32_t s, const float *pos, const float *vel)
 {
     if (s >= scenes.size() || !pos)
         return fail(SPH_E_INVALID, "upload: bad scene or NULL positions");
+    if (inject_failure == kFailUpload) {
+        inject_failure = 0;
+        return fail(SPH_E_NOMEM, "upload: injected failure");
+    }
     BatchScene &b = scenes[s];
     const uint32_t n = b.desc.n, p0 = b.P.p_begin;
     scene_grid(b, pos, n);

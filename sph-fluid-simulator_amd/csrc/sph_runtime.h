@@ -29,7 +29,8 @@ int make_grid(StepParams &P, const double lo[3], const double hi[3], double cell
 // for scales in [1 / kMaxReach, 1), 0 (invalid) otherwise.
 int grid_reach(double cell_scale);
 
-// Growable device allocation (hipMalloc on growth only).
+// Growable device allocation (hipMalloc on growth only; a failed growth
+// leaves the buffer as it was).
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;

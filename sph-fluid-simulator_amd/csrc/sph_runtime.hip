@@ -220,14 +220,14 @@ int DevBuf::reserve(size_t want)
 {
     if (want <= bytes)
         return SPH_OK;
-    release();
-    if (want == 0)
-        return SPH_OK;
-    hipError_t e = hipMalloc(&ptr, want);
-    if (e != hipSuccess) {
-        ptr = nullptr;
+    // the new block first: when it cannot be had, the old one (and the
+    // capacity the caller already relies on) stays as it was
+    void *fresh = nullptr;
+    hipError_t e = hipMalloc(&fresh, want);
+    if (e != hipSuccess)
         return hip_fail(e, "hipMalloc");
-    }
+    release();
+    ptr = fresh;
     bytes = want;
     return SPH_OK;
 }

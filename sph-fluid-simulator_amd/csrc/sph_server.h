@@ -38,6 +38,7 @@ struct Server {
     std::map<uint64_t, std::unique_ptr<Request>> requests;
     std::deque<Request *> queue;
     std::vector<Request *> active;  // active[k] is batch.scenes[k]
+    Request retired;  // stands for a batch scene no request owns (state DONE)
     uint64_t next_id = 1, used_bytes = 0;
     std::vector<float> step_ms;  // GPU time of every batch step
     double t0 = 0;
@@ -50,6 +51,7 @@ struct Server {
     static uint64_t bytes_for(const BatchScene &s);
     int submit(const sph_scene_desc_v1 &d, const float *pos, const float *vel, uint32_t steps, uint64_t *id);
     int tick(uint32_t *ran_steps);
+    void requeue(const std::vector<Request *> &join);
     int fetch(uint64_t id, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids);
     int stats(sph_server_stats_v1 *out) const;
 };

@@ -81,6 +81,7 @@ int Server::init(const sph_server_config_v1 *c)
         cfg.max_steps_per_tick = 8;
     cfg.max_steps_per_tick = std::min<uint32_t>(cfg.max_steps_per_tick, 256u);
     batch.use_graph = (cfg.flags & SPH_SERVER_NO_GRAPH) == 0;
+    retired.state = SPH_REQ_DONE;
     batch.max_capacity =
         std::min<uint64_t>(cfg.memory_budget / (kBytesPerParticle + (kBytesPerWave + kBytesPerPartTask) / 64u),
                            kMaxParticles);
@@ -127,12 +128,26 @@ int Server::submit(const sph_scene_desc_v1 &d, const float *pos, const float *ve
     return SPH_OK;
 }
 
+// Puts the requests of a failed admission back at the head of the queue,
+// in their order, and releases their charge.
+void Server::requeue(const std::vector<Request *> &join)
+{
+    for (auto it = join.rbegin(); it != join.rend(); ++it) {
+        (*it)->state = SPH_REQ_QUEUED;
+        used_bytes -= (*it)->bytes;
+        queue.push_front(*it);
+    }
+}
+
 int Server::tick(uint32_t *ran_steps)
 {
     int rc;
     if (ran_steps)
         *ran_steps = 0;
-    // Retired scenes leave and queued requests join in one relayout.
+    // Retired scenes leave and queued requests join in one relayout.  The
+    // admission is committed only once the relayout and every upload
+    // succeeded; otherwise the joining requests go back to the head of the
+    // queue and the batch keeps the scenes that were running.
     std::vector<Request *> keep, join;
     for (Request *r : active)
         if (r->state == SPH_REQ_ACTIVE)
@@ -156,19 +171,35 @@ int Server::tick(uint32_t *ran_steps)
             next.push_back(r->scene);
             from.push_back(-1);
         }
-        if ((rc = batch.relayout(std::move(next), from)))
+        if ((rc = batch.relayout(std::move(next), from))) {
+            requeue(join);  // the batch still holds the old scenes, as `active` says
             return rc;
+        }
+        active = keep;
+        active.insert(active.end(), join.begin(), join.end());  // matches batch.scenes
         for (size_t k = 0; k < join.size(); ++k) {
-            Request *r = join[k];
-            if ((rc = batch.upload((uint32_t)(keep.size() + k), r->pos.data(), r->vel.data())))
+            if ((rc = batch.upload((uint32_t)(keep.size() + k), join[k]->pos.data(), join[k]->vel.data()))) {
+                // take the joined scenes out again: the kept ones move down
+                // to their new slots (no growth, so only device copies)
+                std::vector<BatchScene> back(batch.scenes.begin(), batch.scenes.begin() + (ptrdiff_t)keep.size());
+                std::vector<int> idx(keep.size());
+                for (size_t q = 0; q < keep.size(); ++q)
+                    idx[q] = (int)q;
+                if (batch.relayout(std::move(back), idx) == SPH_OK)
+                    active = keep;
+                else  // the joined scenes stay in the batch, marked retired
+                    for (size_t q = keep.size(); q < active.size(); ++q)
+                        active[q] = &retired;
+                requeue(join);
                 return rc;
+            }
+        }
+        for (Request *r : join) {
             std::vector<float>().swap(r->pos);
             std::vector<float>().swap(r->vel);
             r->state = SPH_REQ_ACTIVE;
             r->t_admit = now();
         }
-        active = keep;
-        active.insert(active.end(), join.begin(), join.end());
     }
     if (active.empty())
         return SPH_OK;
@@ -341,6 +372,14 @@ int sph_server_fetch(sph_server *s, uint64_t request_id, float *pos, float *vel,
     if (!s)
         return fail(SPH_E_INVALID, "fetch: NULL server");
     return s->s.fetch(request_id, pos, vel, rho, pressure, ids);
+}
+
+int sph_server_inject_failure(sph_server *s, int stage)
+{
+    if (!s || (stage != SPH_SERVER_FAIL_RELAYOUT && stage != SPH_SERVER_FAIL_UPLOAD))
+        return fail(SPH_E_INVALID, "inject_failure: NULL server or unknown stage %d", stage);
+    s->s.batch.inject_failure = stage == SPH_SERVER_FAIL_RELAYOUT ? Batch::kFailRelayout : Batch::kFailUpload;
+    return SPH_OK;
 }
 
 int sph_server_stats(const sph_server *s, sph_server_stats_v1 *out)

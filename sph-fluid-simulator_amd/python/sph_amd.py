@@ -29,6 +29,8 @@ SPH_E_NOMEM = -3
 SPH_E_HIP = -4
 SPH_E_LIMIT = -5
 SPH_E_STATE = -6
+SPH_SERVER_FAIL_RELAYOUT = 1
+SPH_SERVER_FAIL_UPLOAD = 2
 SPH_SCENE_KEEP_FORCE = 1
 SPH_SCENE_LANE_WALK = 2
 SPH_SERVER_NO_GRAPH = 1
@@ -164,6 +166,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_server_status": (c_int, [c_void_p, c_uint64, POINTER(c_int), POINTER(c_uint32),
                                       POINTER(c_double)]),
         "sph_server_fetch": (c_int, [c_void_p, c_uint64] + [c_void_p] * 5),
+        "sph_server_inject_failure": (c_int, [c_void_p, c_int]),
         "sph_server_stats": (c_int, [c_void_p, POINTER(ServerStats)]),
         "sph_server_step_times": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
     }
@@ -505,6 +508,10 @@ class Server:
         st = ServerStats()
         _check(self._lib.sph_server_stats(self._h, ctypes.byref(st)))
         return st.as_dict()
+
+    def inject_failure(self, stage: int):
+        """Tests only: the next relayout (1) or upload (2) fails once."""
+        _check(self._lib.sph_server_inject_failure(self._h, stage))
 
     def step_times(self) -> np.ndarray:
         n = c_size_t(0)

@@ -97,3 +97,50 @@ def test_server_stream_smoke(sph):
     assert res["completed"] >= 1
     assert res["batch_steps"] > 0 and res["step_ms_p99"] >= res["step_ms_p50"] > 0
     assert res["particle_updates_per_s"] > 0
+
+
+@pytest.mark.parametrize("stage", ["relayout", "upload"])
+def test_server_failed_admission_requeues_and_keeps_running_scenes(sph, stage):
+    """A tick whose admission fails (injected at the relayout's last fallible
+    point, or at the upload of the first joining scene) returns the error,
+    puts the joining requests back at the head of the queue with their
+    memory charge released, and leaves the running scene untouched; the next
+    tick admits them and every result still equals the scene stepped alone.
+    The first admission grows the batch from nothing, so the relayout case
+    covers the grow path's roll-back; the second admission joins a running
+    scene."""
+    code = sph.SPH_SERVER_FAIL_RELAYOUT if stage == "relayout" else sph.SPH_SERVER_FAIL_UPLOAD
+    scenes = [sph_scenes.request_scene(4000, "dambreak", 21), sph_scenes.request_scene(7000, "droplet", 22),
+              sph_scenes.request_scene(3000, "dambreak", 23)]
+    steps = [12, 9, 7]
+    with sph.Server(max_steps_per_tick=3) as srv:
+        # first admission: two joining scenes on an empty batch
+        a = srv.submit(sph_serving.scene_desc(scenes[0]), scenes[0].pos, scenes[0].vel, steps[0])
+        b = srv.submit(sph_serving.scene_desc(scenes[1]), scenes[1].pos, scenes[1].vel, steps[1])
+        srv.inject_failure(code)
+        with pytest.raises(sph.SphError):
+            srv.tick()
+        st = srv.stats()
+        assert st["queued"] == 2 and st["active"] == 0 and st["used_bytes"] == 0, st
+        assert [srv.status(r)[0] for r in (a, b)] == [sph.SPH_REQ_QUEUED] * 2
+        srv.tick()
+        assert [srv.status(r)[:2] for r in (a, b)] == [(sph.SPH_REQ_ACTIVE, 3)] * 2
+        charged = srv.stats()["used_bytes"]
+        # second admission: joins the two running scenes
+        c = srv.submit(sph_serving.scene_desc(scenes[2]), scenes[2].pos, scenes[2].vel, steps[2])
+        srv.inject_failure(code)
+        with pytest.raises(sph.SphError):
+            srv.tick()
+        st = srv.stats()
+        assert st["queued"] == 1 and st["active"] == 2 and st["used_bytes"] == charged, st
+        assert srv.status(c)[0] == sph.SPH_REQ_QUEUED
+        assert [srv.status(r)[:2] for r in (a, b)] == [(sph.SPH_REQ_ACTIVE, 3)] * 2
+        ids = [a, b, c]
+        for _ in range(50):
+            if all(srv.status(r)[0] == sph.SPH_REQ_DONE for r in ids):
+                break
+            srv.tick()
+        got = [srv.fetch(r) for r in ids]
+        assert srv.stats()["used_bytes"] == 0
+    for k, sc in enumerate(scenes):
+        assert _same(got[k], _alone(sph, sc, steps[k])), f"request {k} ({sc.name}) differs from alone"
