@@ -138,6 +138,24 @@ int sph_hip_update_particles_box(sph_particle_v1 *particles, float *transforms,
                                  size_t n, const sph_settings_v1 *settings,
                                  float dt, const sph_box_v1 *box);
 
+/* Largest n sph_hip_update_particles_noalloc accepts. */
+#define SPH_NOALLOC_MAX_PARTICLES 65536u
+
+/* As sph_hip_update_particles_box, but allocates no device memory: every
+ * device buffer of the step comes from one fixed 64 MiB array in device
+ * memory, which each GPU the library runs on holds until the process ends.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (harnesses that forbid hipMalloc, allocation-free serving loops).
+ * n > SPH_NOALLOC_MAX_PARTICLES fails with SPH_E_LIMIT.  Results are
+ * bit-identical to sph_hip_update_particles_box.  All calls on one GPU share
+ * its array: the library serialises them with a process-wide lock and each
+ * returns after its device work finished; other device work that wrote into
+ * the array during a call would race with it.  Runs on the null stream. */
+int sph_hip_update_particles_noalloc(sph_particle_v1 *particles,
+                                     float *transforms, size_t n,
+                                     const sph_settings_v1 *settings,
+                                     float dt, const sph_box_v1 *box);
+
 /* Neighbour multisets of the reference scan for the given particles (the
  * sets parallelDensityAndPressures sums over, src/sph.cpp:40-66), computed
  * by the density kernel's own candidate walk.  offsets has n + 1 entries;

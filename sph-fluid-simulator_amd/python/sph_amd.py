@@ -31,6 +31,7 @@ SPH_E_LIMIT = -5
 SPH_E_STATE = -6
 SPH_SERVER_FAIL_RELAYOUT = 1
 SPH_SERVER_FAIL_UPLOAD = 2
+NOALLOC_MAX_PARTICLES = 65536  # SPH_NOALLOC_MAX_PARTICLES
 SPH_SCENE_KEEP_FORCE = 1
 SPH_SCENE_LANE_WALK = 2
 SPH_SERVER_NO_GRAPH = 1
@@ -128,6 +129,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_classify_host": (c_int, [POINTER(c_float), c_float, POINTER(ctypes.c_int32)]),
         "sph_hip_update_particles": (c_int, [c_void_p, c_void_p, c_size_t, POINTER(Settings), c_float]),
         "sph_hip_update_particles_box": (
+            c_int, [c_void_p, c_void_p, c_size_t, POINTER(Settings), c_float, POINTER(Box)]),
+        "sph_hip_update_particles_noalloc": (
             c_int, [c_void_p, c_void_p, c_size_t, POINTER(Settings), c_float, POINTER(Box)]),
         "sph_hip_neighbors": (
             c_int, [c_void_p, c_size_t, POINTER(Settings), c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
@@ -252,6 +255,18 @@ def update_particles(particles: np.ndarray, s: Settings, dt: float, box_: Box | 
     else:
         rc = lib.sph_hip_update_particles_box(_ptr(p), _ptr(xf), n, ctypes.byref(s), dt, ctypes.byref(box_))
     _check(rc)
+    return p, xf
+
+
+def update_particles_noalloc(particles: np.ndarray, s: Settings, dt: float, box_: Box,
+                             transforms: bool = True):
+    """As update_particles with a box, through the entry point that
+    allocates no device memory (n <= NOALLOC_MAX_PARTICLES)."""
+    p = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE).copy()
+    n = len(p)
+    xf = np.zeros((n, 16), dtype=np.float32) if transforms else None
+    _check(load().sph_hip_update_particles_noalloc(_ptr(p), _ptr(xf), n, ctypes.byref(s), dt,
+                                                   ctypes.byref(box_)))
     return p, xf
 
 
