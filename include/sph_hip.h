@@ -187,6 +187,18 @@ int sph_hip_download(sph_ctx *ctx, float *pos, float *vel, float *rho,
  * may be NULL).  hash/density/pressure/force are from the last step. */
 int sph_hip_download_particles(sph_ctx *ctx, sph_particle_v1 *out,
                                float *transforms);
+/* Tests: how the production walks of the next step see the current state.
+ * Runs that step's passes up to density (cell sort, part plan, union /
+ * per-lane / split walks, neighbour lists) with a neighbour count switched
+ * on, without changing the state or the results of later steps.  Per
+ * particle id: count[i] = neighbours with multiplicity (the terms the
+ * reference sums, src/sph.cpp:40-66); walk[i] (may be NULL) = bit 0: its
+ * wave task walks the union of its lanes' stencils, bit 1: the task is
+ * split into parts, bit 2: a neighbour list was recorded; list_count[i]
+ * (may be NULL) = list entries or 0xFFFFFFFF; list (may be NULL): 64
+ * entries per id, the listed neighbours' ids (each once) in walk order. */
+int sph_hip_walk_dump(sph_ctx *ctx, uint32_t *count, uint32_t *walk,
+                      uint32_t *list_count, uint32_t *list);
 /* 1 = replay steps through a captured HIP graph (default), 0 = launch. */
 int sph_hip_set_graph(sph_ctx *ctx, int enable);
 uint32_t sph_hip_count(const sph_ctx *ctx);
@@ -210,6 +222,9 @@ int sph_batch_sync(sph_batch *b);
 int sph_batch_download(sph_batch *b, uint32_t scene, float *pos, float *vel,
                        float *rho, float *pressure, uint32_t *ids);
 int sph_batch_set_graph(sph_batch *b, int enable);
+/* sph_hip_walk_dump for one scene of a batch. */
+int sph_batch_walk_dump(sph_batch *b, uint32_t scene, uint32_t *count,
+                        uint32_t *walk, uint32_t *list_count, uint32_t *list);
 uint32_t sph_batch_scene_count(const sph_batch *b);
 int sph_batch_profile(sph_batch *b, int nsteps, double *ms, int max_k,
                       int *nk, char *names_buf, size_t names_cap);

@@ -346,40 +346,46 @@ void FN(density_pressure)(particle *p, size_t n, const uint32_t *table,
  *   pressure = ((((-dir) * m) * (p_i + p_j)) / (2 rho_j)) * spikyGrad,
  *              then *= (float)pow(h - dist, 2)
  *   visco    = (((mu * m) * (dv / rho_j)) * spikyLap) * (h - dist)        */
+static void force_of(const particle *p, size_t n, const uint32_t *table, const settings *s, size_t i,
+                     real f[3])
+{
+    const particle *pi = &p[i];
+    f[0] = f[1] = f[2] = 0;
+    FOR_EACH_NEIGHBOR(p, n, table, s, i, pi, pj, dist2, {
+        real dist = SQRT_R(dist2);
+        real v[3] = {pj->position[0] - pi->position[0],
+                     pj->position[1] - pi->position[1],
+                     pj->position[2] - pi->position[2]};
+        real dd = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        real inv = (real)1 / SQRT_R(dd);
+        real psum = pi->pressure + pj->pressure;
+        real den = 2 * pj->density;
+        real hd = s->h - dist;
+        real sq = (real)pow(hd, 2);
+        real mum = s->viscosity * s->mass;
+        for (int k = 0; k < 3; k++) {
+            real dir = v[k] * inv;
+            real pf = (((-dir) * s->mass) * psum) / den * s->spikyGrad;
+            pf *= sq;
+            f[k] += pf;
+        }
+        for (int k = 0; k < 3; k++) {
+            real dv = pj->velocity[k] - pi->velocity[k];
+            real vf = ((mum * (dv / pj->density)) * s->spikyLap) * hd;
+            f[k] += vf;
+        }
+    });
+}
+
 static void forces_range(void *vctx, size_t start, size_t end)
 {
     pass_ctx *c = (pass_ctx *)vctx;
-    const settings *s = c->s;
     for (size_t i = start; i < end; i++) {
-        particle *pi = &c->p[i];
-        real f[3] = {0, 0, 0};
-        FOR_EACH_NEIGHBOR(c->p, c->n, c->table, s, i, pi, pj, dist2, {
-            real dist = SQRT_R(dist2);
-            real v[3] = {pj->position[0] - pi->position[0],
-                         pj->position[1] - pi->position[1],
-                         pj->position[2] - pi->position[2]};
-            real dd = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            real inv = (real)1 / SQRT_R(dd);
-            real psum = pi->pressure + pj->pressure;
-            real den = 2 * pj->density;
-            real hd = s->h - dist;
-            real sq = (real)pow(hd, 2);
-            real mum = s->viscosity * s->mass;
-            for (int k = 0; k < 3; k++) {
-                real dir = v[k] * inv;
-                real pf = (((-dir) * s->mass) * psum) / den * s->spikyGrad;
-                pf *= sq;
-                f[k] += pf;
-            }
-            for (int k = 0; k < 3; k++) {
-                real dv = pj->velocity[k] - pi->velocity[k];
-                real vf = ((mum * (dv / pj->density)) * s->spikyLap) * hd;
-                f[k] += vf;
-            }
-        });
-        pi->force[0] = f[0];
-        pi->force[1] = f[1];
-        pi->force[2] = f[2];
+        real f[3];
+        force_of(c->p, c->n, c->table, c->s, i, f);
+        c->p[i].force[0] = f[0];
+        c->p[i].force[1] = f[1];
+        c->p[i].force[2] = f[2];
     }
 }
 
@@ -497,6 +503,40 @@ size_t FN(neighbors)(const particle *sorted, size_t n, const uint32_t *table,
             offsets[i + 1] = total;
     }
     return total;
+}
+
+/* Neighbour rows (sorted indices, each repeated m_ij times) of selected
+ * sorted indices only: CSR offsets[k + 1]; returns the total, writing at
+ * most cap indices. */
+size_t FN(neighbors_subset)(const particle *sorted, size_t n, const uint32_t *table,
+                            const settings *s, const uint32_t *which, size_t k,
+                            uint64_t *offsets, uint32_t *idx, size_t cap)
+{
+    size_t total = 0;
+    if (offsets)
+        offsets[0] = 0;
+    for (size_t a = 0; a < k; a++) {
+        const size_t i = which[a];
+        const particle *pi = &sorted[i];
+        FOR_EACH_NEIGHBOR(sorted, n, table, s, i, pi, pj, dist2, {
+            if (idx && total < cap)
+                idx[total] = (uint32_t)(pj - sorted);
+            total++;
+        });
+        if (offsets)
+            offsets[a + 1] = total;
+    }
+    return total;
+}
+
+/* Force of selected sorted indices (k x 3), from the densities and
+ * pressures the particles hold. */
+void FN(forces_subset)(const particle *sorted, size_t n, const uint32_t *table,
+                       const settings *s, const uint32_t *which, size_t k,
+                       real *force_out)
+{
+    for (size_t a = 0; a < k; a++)
+        force_of(sorted, n, table, s, which[a], force_out + 3 * a);
 }
 
 /* Density (with self density) of selected sorted indices, plus the number

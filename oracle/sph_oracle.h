@@ -111,7 +111,14 @@ typedef struct {
     void P##_density_subset(const P##_particle *sorted, size_t n,              \
                             const uint32_t *table, const P##_settings *s,      \
                             const uint32_t *which, size_t k, T *rho_out,       \
-                            uint32_t *count_out);
+                            uint32_t *count_out);                      \
+    size_t P##_neighbors_subset(const P##_particle *sorted, size_t n,          \
+                                const uint32_t *table, const P##_settings *s,  \
+                                const uint32_t *which, size_t k,               \
+                                uint64_t *offsets, uint32_t *idx, size_t cap); \
+    void P##_forces_subset(const P##_particle *sorted, size_t n,               \
+                           const uint32_t *table, const P##_settings *s,       \
+                           const uint32_t *which, size_t k, T *force_out);
 
 ORC_DECLARE(orc32, float)
 ORC_DECLARE(orc64, double)

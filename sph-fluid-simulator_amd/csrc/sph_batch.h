@@ -76,6 +76,10 @@ struct Batch {
     int download(uint32_t s, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids);
     int download_particles(uint32_t s, sph_particle_v1 *out, float *transforms);
     int profile(int nsteps, double *ms, int max_k, int *nk, char *names_buf, size_t names_cap);
+    // Test view of the production walks on the current state (see
+    // sph_batch_walk_dump): runs the step's passes up to density with the
+    // neighbour count switched on, into temporary B-side arrays.
+    int walk_dump(uint32_t s, uint32_t *count, uint32_t *walk, uint32_t *list_count, uint32_t *list);
     void drop_graph();
 };
 

@@ -607,6 +607,88 @@ int Batch::profile(int nsteps, double *ms, int max_k, int *nk, char *names_buf, 
     return SPH_OK;
 }
 
+int Batch::walk_dump(uint32_t s, uint32_t *count, uint32_t *walk, uint32_t *list_count, uint32_t *list)
+{
+    int rc;
+    if (s >= scenes.size() || !count)
+        return fail(SPH_E_INVALID, "walk_dump: bad scene or NULL count");
+    if ((rc = prepare()))
+        return rc;
+    const uint32_t N = total_n;
+    // The B-side arrays hold the last step's order and densities, which the
+    // downloads read: the dump's own go to temporaries.
+    DevBuf tpos, tvel, tcell, tperm, trho, tposR, tvelR, tcount;
+    if ((rc = tpos.reserve(16ull * N)) || (rc = tvel.reserve(16ull * N)) || (rc = tcell.reserve(4ull * N)) ||
+        (rc = tperm.reserve(4ull * N)) || (rc = trho.reserve(4ull * N)) || (rc = tposR.reserve(16ull * N)) ||
+        (rc = tvelR.reserve(16ull * N)) || (rc = tcount.reserve(4ull * N)))
+        return rc;
+    ParticleArrays a = w.particles();
+    a.posB = (float4 *)tpos.ptr;
+    a.velB = (float4 *)tvel.ptr;
+    a.cellcB = (uint32_t *)tcell.ptr;
+    a.perm = (uint32_t *)tperm.ptr;
+    a.rhoB = (float *)trho.ptr;
+    a.posR = (float4 *)tposR.ptr;
+    a.velR = (float4 *)tvelR.ptr;
+    const CellArrays c = w.cells();
+    const SceneSet S = w.scene_set();
+    const PartPlan pp = w.part_plan();
+    const NeighborLists nl = w.lists();
+    launch_scan(stream, c, false);
+    keys_dirty = true;  // the scan consumed the cell counts: the next step recomputes the keys
+    launch_scatter(stream, a, c, S);
+    launch_sort_big_cells(stream, a, c);
+    launch_reorder(stream, a, c, S);
+    launch_plan_parts(stream, a, c, S, pp);
+    NeighborDump d{(uint32_t *)tcount.ptr, nullptr, nullptr, true};
+    launch_density(stream, a, c, S, &d, &pp, kStencilBlock, &nl);
+
+    const BatchScene &b = scenes[s];
+    const uint32_t n = b.desc.n, p0 = b.P.p_begin;
+    uint32_t t0 = 0;  // first wave task of scene s (tasks follow the scenes in order)
+    for (uint32_t k = 0; k < s; ++k)
+        t0 += (scenes[k].desc.n + 63u) / 64u;
+    const uint32_t nt = (n + 63u) / 64u;
+    std::vector<uint32_t> cnt(n), lc(n), off(nt + 1), ids(n);
+    std::vector<float4> vb(n);
+    std::vector<int4> wk(2ull * nt);
+    std::vector<uint32_t> lst(list ? (size_t)kListMax * n : 0);
+    hipError_t e = hipMemcpyAsync(cnt.data(), (uint32_t *)tcount.ptr + p0, 4ull * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(lc.data(), nl.count + p0, 4ull * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(vb.data(), a.velB + p0, 16ull * n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(off.data(), pp.off + t0, 4ull * (nt + 1), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(wk.data(), pp.walk + 2ull * t0, sizeof(int4) * 2ull * nt, hipMemcpyDeviceToHost, stream);
+    for (uint32_t g = 0; list && g < kListMax / 4u && e == hipSuccess; ++g)  // groups of four entries
+        e = hipMemcpyAsync(lst.data() + 4ull * g * n, nl.list + list_at(4u * g, p0, nl.stride), 16ull * n,
+                           hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    if (e == hipSuccess)
+        e = hipGetLastError();
+    if (e != hipSuccess)
+        return hip_fail(e, "walk_dump");
+    for (uint32_t q = 0; q < n; ++q)
+        std::memcpy(&ids[q], &vb[q].w, 4);
+    for (uint32_t q = 0; q < n; ++q) {
+        const uint32_t id = ids[q], t = q / 64u;
+        if (id >= n)
+            return fail(SPH_E_HIP, "walk_dump: slot %u holds id %u of %u", q, id, n);
+        count[id] = cnt[q];
+        if (walk)
+            walk[id] = (wk[2 * t + 1].w ? 1u : 0u) | (off[t + 1] != off[t] ? 2u : 0u) | (lc[q] != kNoList ? 4u : 0u);
+        if (list_count)
+            list_count[id] = lc[q];
+        if (list && lc[q] != kNoList)
+            for (uint32_t k = 0; k < lc[q]; ++k)
+                list[(size_t)id * kListMax + k] = ids[lst[list_at(k, q, n)] - p0];
+    }
+    return SPH_OK;
+}
+
 }  // namespace sph
 
 using namespace sph;
@@ -714,6 +796,21 @@ int sph_hip_profile(sph_ctx *c, int nsteps, double *ms, int max_k, int *nk, char
 }
 
 // ---- batch of scenes
+
+int sph_hip_walk_dump(sph_ctx *c, uint32_t *count, uint32_t *walk, uint32_t *list_count, uint32_t *list)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "walk_dump: NULL context");
+    return c->b.walk_dump(0, count, walk, list_count, list);
+}
+
+int sph_batch_walk_dump(sph_batch *b, uint32_t scene, uint32_t *count, uint32_t *walk, uint32_t *list_count,
+                        uint32_t *list)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "walk_dump: NULL batch");
+    return b->b.walk_dump(scene, count, walk, list_count, list);
+}
 
 int sph_batch_create(const sph_scene_desc_v1 *descs, uint32_t nscenes, sph_batch **out)
 {

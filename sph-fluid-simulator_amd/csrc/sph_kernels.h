@@ -6,7 +6,7 @@
 //     posA   float4  x, y, z, w = bits of the 16-bit bucket H16(getCell(x))
 //     velA   float4  vx, vy, vz, w = bits of the particle id
 //     cellcA u32     packed grid cell + slow-multiplicity flag
-//     rankA  u32     arrival rank of the particle inside its grid cell
+//     rankA  u32     scratch (the drop-in output's 16-bit bucket ranks)
 //   "B" arrays: the same particles in grid-cell order (stable by A index)
 //     posB, velB, cellcB  as above
 //     rhoB   f32     density                          (density pass)
@@ -14,7 +14,8 @@
 //     velR   float4  vx, vy, vz, pressure             (density pass)
 //   posR / velR are what the force pass gathers: one record for the
 //   candidate filter and one more per neighbour.
-//   cells: counts[ncells] (zero between steps), cstart[ncells + 1]
+//   cells: counts[ncells] (between steps: the particles of A per cell; the
+//          scatter counts them back to zero), cstart[ncells + 1]
 //
 // Several independent scenes can share the arrays ("batch"): scene s owns
 // the slot range [p_begin, p_end) of both orders and the cell range
@@ -57,7 +58,7 @@ struct ParticleArrays {
 };
 
 struct CellArrays {
-    uint32_t *counts;  // ncells_padded entries, zero between steps
+    uint32_t *counts;  // ncells_padded entries: A's particles per cell between steps
     uint32_t *cstart;  // ncells_padded + 1 entries
     uint32_t *partials;
     uint32_t *big;     // [0] = number of big cells, then their indices (may be null)
@@ -125,11 +126,37 @@ void launch_plan_parts(hipStream_t st, const ParticleArrays &a, const CellArrays
 
 // Debug outputs of the density pass: per-particle neighbour count (with
 // multiplicity) and, when out != nullptr, the neighbour ids of row s at
-// out[off[s] ...].
+// out[off[s] ...].  With `production` set, only the count is written, by
+// the production walks of the step (part plan, union / per-lane / split
+// parts, neighbour lists) instead of a separate whole-stencil walk.
 struct NeighborDump {
     uint32_t *count;
     const uint64_t *off;
     uint32_t *out;
+    bool production = false;
+};
+
+// Neighbour lists.  The density pass records, for every particle of an
+// unsplit wave task that walks its candidates per lane, the slots of its
+// neighbours (each once, in walk order: rows z-major, then y, slots
+// ascending) when there are at most kListMax of them; the force pass then
+// visits exactly those slots instead of filtering the stencil again.  The
+// order is the per-lane walk's, so the force sums are the same bit for bit.
+// Entries go in groups of four: entry k of slot s lives at
+// list[4 ((k / 4) stride + s) + k % 4], so one 16-byte load gives a lane
+// four neighbours and a wave's loads of one group are contiguous.
+constexpr uint32_t kListMax = 64;
+constexpr uint32_t kNoList = 0xFFFFFFFFu;  // count of a slot without a list
+
+__host__ __device__ inline size_t list_at(uint32_t k, uint32_t s, uint32_t stride)
+{
+    return 4ull * ((size_t)(k >> 2) * stride + s) + (k & 3u);
+}
+
+struct NeighborLists {
+    uint32_t *list;   // kListMax * stride entries; null: no lists
+    uint32_t *count;  // per slot: list entries, or kNoList
+    uint32_t stride;  // slot capacity of the arrays
 };
 
 inline uint32_t blocks_for(uint32_t n, int block = kBlock) { return (n + block - 1) / block; }
@@ -139,7 +166,8 @@ void launch_unpack_aos(hipStream_t st, const void *aos, uint32_t n, float4 *posA
 void launch_bbox(hipStream_t st, const float4 *pos, uint32_t n, int32_t *bbox6);
 void launch_init_keys(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S);
 // Scan of the cell counts; also lists the cells above kSmallCell in c.big.
-void launch_scan(hipStream_t st, const CellArrays &c);
+// clear_counts: zero the counts (else the scatter counts them back down).
+void launch_scan(hipStream_t st, const CellArrays &c, bool clear_counts);
 void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S);
 // Sorts the A indices of every listed big cell (deterministic in-cell order).
 void launch_sort_big_cells(hipStream_t st, const ParticleArrays &a, const CellArrays &c);
@@ -148,11 +176,15 @@ void launch_reorder(hipStream_t st, const ParticleArrays &a, const CellArrays &c
 // dump modes always use kStencilBlock); results do not depend on it.
 // `parts` (may be null: every wave task unsplit, one wave each) is the
 // plan launch_plan_parts made for this step.
+// `lists` (may be null: none) receives the neighbour lists of the step;
+// the force pass reads the same.
 void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                    const NeighborDump *dump, const PartPlan *parts = nullptr, int block = kStencilBlock);
+                    const NeighborDump *dump, const PartPlan *parts = nullptr, int block = kStencilBlock,
+                    const NeighborLists *lists = nullptr);
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
                             const SceneSet &S, float4 *force_out, bool next_keys,
-                            const PartPlan *parts = nullptr, int block = kStencilBlock);
+                            const PartPlan *parts = nullptr, int block = kStencilBlock,
+                            const NeighborLists *lists = nullptr);
 // Drop-in output (single scene): order slots by the pre-step 16-bit bucket
 // (ties by id), and write the reference AoS layout plus transforms.
 void launch_h16_rank(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n);

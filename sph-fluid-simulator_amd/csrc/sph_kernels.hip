@@ -202,10 +202,10 @@ void launch_bbox(hipStream_t st, const float4 *pos, uint32_t n, int32_t *bbox6)
     hipLaunchKernelGGL(k_bbox, dim3(blocks_for(n)), dim3(kBlock), 0, st, pos, n, bbox6);
 }
 
-// Cell key, bucket and in-cell arrival rank of the positions in posA.
+// Cell key and bucket of the positions in posA, counted into their cells
+// (the count is not read back here: the scatter ranks the particles).
 __global__ __launch_bounds__(kBlock) void k_init_keys(float4 *__restrict__ posA,
                                                      uint32_t *__restrict__ cellcA,
-                                                     uint32_t *__restrict__ rankA,
                                                      uint32_t *__restrict__ counts, SceneSet S)
 {
     Lane L;
@@ -217,20 +217,21 @@ __global__ __launch_bounds__(kBlock) void k_init_keys(float4 *__restrict__ posA,
     Classified c = classify(p.x, p.y, p.z, P);
     posA[i].w = __uint_as_float(c.h16);
     cellcA[i] = c.cellc;
-    rankA[i] = atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
+    atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
 }
 
 void launch_init_keys(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S)
 {
     hipLaunchKernelGGL(k_init_keys, dim3(blocks_for_waves(S.nwaves, kBlock)), dim3(kBlock), 0, st,
-                       a.posA, a.cellcA, a.rankA, c.counts, S);
+                       a.posA, a.cellcA, c.counts, S);
 }
 
 // ------------------------------------------------------------- cell scan
 // Three passes over the (padded) cell array: per-block totals, a one-block
 // scan of the totals, then the block-local exclusive scan plus offset.  The
-// last pass also clears the counts for the next step's atomics and appends
-// every cell above kSmallCell to the big-cell list.
+// last pass appends every cell above kSmallCell to the big-cell list and,
+// when asked, clears the counts (the cell scan leaves them: the scatter
+// counts them back down to zero while it ranks the particles).
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t x)
 {
@@ -304,7 +305,8 @@ __global__ void k_scan_partials(uint32_t *__restrict__ partials, uint32_t nparts
 }
 
 __global__ void k_scan_final(uint32_t *__restrict__ counts, const uint32_t *__restrict__ partials,
-                             uint32_t *__restrict__ cstart, uint32_t *__restrict__ big, uint32_t big_cap)
+                             uint32_t *__restrict__ cstart, uint32_t *__restrict__ big, uint32_t big_cap,
+                             bool clear)
 {
     size_t base = (size_t)blockIdx.x * kScanChunk + (size_t)threadIdx.x * kScanItems;
     uint4 *src = (uint4 *)(counts + base);
@@ -336,7 +338,8 @@ __global__ void k_scan_final(uint32_t *__restrict__ counts, const uint32_t *__re
         q.w = run;
         run += v[4 * k + 3];
         dst[k] = q;
-        src[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (clear)
+            src[k] = make_uint4(0u, 0u, 0u, 0u);
     }
     if (big) {
 #pragma unroll
@@ -350,20 +353,29 @@ __global__ void k_scan_final(uint32_t *__restrict__ counts, const uint32_t *__re
     }
 }
 
-void launch_scan(hipStream_t st, const CellArrays &c)
+void launch_scan(hipStream_t st, const CellArrays &c, bool clear_counts)
 {
     uint32_t nparts = c.ncells_padded / kScanChunk;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kBlock), 0, st, c.counts, c.partials);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, st, c.partials, nparts,
                        c.cstart + c.ncells_padded, c.big);
     hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, st, c.counts, c.partials,
-                       c.cstart, c.big, c.big_cap);
+                       c.cstart, c.big, c.big_cap, clear_counts);
 }
 
 // ---------------------------------------------------------- scatter/reorder
 
+// Each particle takes a distinct place in its cell's range by counting the
+// cell's count down (the order within a cell is fixed afterwards by the
+// reorder), which also leaves the counts at zero for the next step.  The
+// one atomic whose result a thread waits on is here, in a kernel that does
+// nothing else, rather than at the end of the force pass.  Lanes of a wave
+// that share a cell (A is in the previous step's cell order, so a wave
+// holds few cells) take their places with one atomic: the lowest lane of
+// each group subtracts the group's size, and every lane of the group adds
+// its position among them.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t *__restrict__ cellcA,
-                                                   const uint32_t *__restrict__ rankA,
+                                                   uint32_t *__restrict__ counts,
                                                    const uint32_t *__restrict__ cstart,
                                                    uint32_t *__restrict__ perm, SceneSet S)
 {
@@ -371,13 +383,33 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t *__restrict__
     if (!lane_of<kBlock>(S, L))
         return;
     const uint32_t i = L.slot;
-    perm[cstart[linear_cell(*L.P, cellcA[i])] + rankA[i]] = i;
+    const uint32_t lin = linear_cell(*L.P, cellcA[i]);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t rem = __ballot(1);  // the live lanes
+    uint32_t leader = lane, size = 1, offset = 0;
+    while (rem != 0ull) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(rem);
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)lin, (int)l);
+        const uint64_t group = __ballot(lin == c) & rem;
+        if (lin == c) {
+            leader = l;
+            size = (uint32_t)__builtin_popcountll(group);
+            offset = (uint32_t)__builtin_popcountll(group & below);
+        }
+        rem &= ~group;
+    }
+    uint32_t top = 0;
+    if (lane == leader)
+        top = atomicSub(&counts[lin], size);  // one instruction for every group of the wave
+    top = (uint32_t)__shfl((int)top, (int)leader, 64);
+    perm[cstart[lin] + top - size + offset] = i;
 }
 
 void launch_scatter(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S)
 {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for_waves(S.nwaves, kBlock)), dim3(kBlock), 0, st,
-                       a.cellcA, a.rankA, c.cstart, a.perm, S);
+                       a.cellcA, c.counts, c.cstart, a.perm, S);
 }
 
 // Big cells: one block sorts the A indices of one listed cell (bitonic,
@@ -973,7 +1005,7 @@ void launch_plan_parts(hipStream_t st, const ParticleArrays &a, const CellArrays
     cs.big = nullptr;
     cs.big_cap = 0;
     cs.ncells_padded = pp.padded;
-    launch_scan(st, cs);  // also clears nparts for the next step
+    launch_scan(st, cs, /*clear_counts=*/true);  // nparts must be zero for the next step
     hipLaunchKernelGGL(k_emit_parts, dim3(blocks_for(S.nwaves)), dim3(kBlock), 0, st, pp.off, S.nwaves, pp.tasks,
                        pp.queue, pp.total, pp.nscenes);
 }
@@ -1057,15 +1089,18 @@ __device__ inline bool combine_parts(const PartPlan &pp, uint32_t w, uint32_t pa
 
 // --------------------------------------------------------------- density
 // MODE 0: density + pressure.  MODE 1: also the neighbour count per
-// particle.  MODE 2: also the neighbour ids (multiplicity-repeated).
-// Writes rho, posR = (x, 0.5 / rho) and velR = (v, p) for the force pass.
+// particle.  MODE 2: also the neighbour ids (multiplicity-repeated).  MODE
+// 3: MODE 0 plus the neighbour count (with multiplicity) of the production
+// walks.  Writes rho, posR = (x, 0.5 / rho) and velR = (v, p) for the force
+// pass, and in MODE 0 / 3 the neighbour lists of unsplit per-lane tasks.
 template <int MODE, int BLOCK>
 __global__ __launch_bounds__(BLOCK, 6) void k_density(
     const float4 *__restrict__ pos, const float4 *__restrict__ vel,
     const uint32_t *__restrict__ cellc, const uint32_t *__restrict__ cstart,
     float *__restrict__ rho, float4 *__restrict__ posR, float4 *__restrict__ velR, SceneSet S,
-    NeighborDump dump, PartPlan pp)
+    NeighborDump dump, PartPlan pp, NeighborLists nl)
 {
+    constexpr bool kCount = MODE >= 1;
     for_each_part<BLOCK>(S, pp, pp.queue, [&](uint32_t wt, uint32_t part, uint32_t nparts) {
         Lane L;
         bool active;
@@ -1079,8 +1114,10 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         const float h2 = P.h2, mp6 = P.mp6;
         float acc = 0.0f;
         uint32_t cnt = 0;
+        uint32_t listed = kNoList;
         uint64_t w = (MODE == 2 && active) ? dump.off[s] : 0;
-        auto body = [&](uint32_t j, float4 q) {
+        // true when q passes (and then adds its term)
+        auto body = [&](uint32_t j, float4 q) -> bool {
             float dx, dy, dz;
             float d2 = dist2_ref(pi, q, dx, dy, dz);
             if (d2 < h2) {
@@ -1093,21 +1130,23 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                 float t = h2 - d2;
                 float term = mp6 * ((t * t) * t);
                 acc += (float)m * term;
-                if (MODE >= 1)
+                if (kCount)
                     cnt += (uint32_t)m;
                 if (MODE == 2 && active) {
                     uint32_t id = __float_as_uint(vel[j].w);
                     for (int k = 0; k < m; ++k)
                         dump.out[w++] = id;
                 }
+                return true;
             }
+            return false;
         };
         // Waves with a slow-multiplicity lane (rare where the union pays)
         // take the per-lane walk, so the union walk needs no m_ij.  A split
         // task walks its slice [part, part + 1) / nparts of the sequence.
         UnionRows U;
         uint32_t ucount = 0;
-        if (MODE == 0 && walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {
+        if ((MODE == 0 || MODE == 3) && walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {
             // Wave-uniform candidate j: scalar loads, four records at a time
             // (density is too short per candidate to feed from LDS).
             const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
@@ -1123,6 +1162,8 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                 if (decltype(self_tag)::value)
                     t = (j != s) ? t : 0.0f;
                 acc = acc + mp6 * ((t * t) * t);
+                if (kCount)
+                    cnt += t > 0.0f ? 1u : 0u;
             };
             scalar_union_walk(cstart, U, P, lo, hi, L.first, [&](uint32_t j0, uint32_t j1, auto self_tag) {
                 const float4 *p = pos + j0;
@@ -1137,6 +1178,62 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                 for (; j < j1; ++j, ++p)
                     cand(j, p[0], self_tag);
             });
+        } else if ((MODE == 0 || MODE == 3) && nl.list && nparts == 1u) {
+            // Per-lane walk that also records the lane's neighbours: each
+            // row is filtered in segments of up to 64 candidates into a
+            // bitmask, whose set bits are then appended in slot order.
+            uint32_t nn = 0;
+            // Without slow lanes every neighbour counts once, and the sum
+            // runs branch-free: a failing candidate, or the particle
+            // itself, adds +0, which leaves acc unchanged bit for bit (as
+            // in the union walk), so both forms give the same sums.
+            const bool any_slow = __any(slow);
+            auto fast = [&](uint32_t j, float4 q, uint32_t g) -> uint64_t {
+                float dx, dy, dz;
+                const float d2 = dist2_ref(pi, q, dx, dy, dz);
+                float t = fmaxf(h2 - d2, 0.0f);
+                t = (j != s) ? t : 0.0f;
+                acc = acc + mp6 * ((t * t) * t);
+                if (kCount)
+                    cnt += t > 0.0f ? 1u : 0u;
+                return t > 0.0f ? (1ull << (j - g)) : 0ull;
+            };
+            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+                for (uint32_t g = b; g < e; g += 64u) {
+                    const uint32_t ge = min(g + 64u, e);
+                    uint64_t mask = 0;
+                    uint32_t j = g;
+                    if (!any_slow) {
+                        for (; j + 4u <= ge; j += 4u) {  // four candidates in flight
+                            const float4 q0 = pos[j], q1 = pos[j + 1u], q2 = pos[j + 2u], q3 = pos[j + 3u];
+                            mask |= fast(j, q0, g);
+                            mask |= fast(j + 1u, q1, g);
+                            mask |= fast(j + 2u, q2, g);
+                            mask |= fast(j + 3u, q3, g);
+                        }
+                        for (; j < ge; ++j)
+                            mask |= fast(j, pos[j], g);
+                    }
+                    for (; j + 1u < ge; j += 2u) {  // two candidates in flight
+                        const float4 q0 = pos[j];
+                        const float4 q1 = pos[j + 1u];
+                        if (j != s && body(j, q0))
+                            mask |= 1ull << (j - g);
+                        if (j + 1u != s && body(j + 1u, q1))
+                            mask |= 2ull << (j - g);
+                    }
+                    if (j < ge && j != s && body(j, pos[j]))
+                        mask |= 1ull << (j - g);
+                    while (mask != 0ull) {
+                        const uint32_t k = (uint32_t)__builtin_ctzll(mask);
+                        mask &= mask - 1ull;
+                        if (nn < kListMax)
+                            nl.list[list_at(nn, s, nl.stride)] = g + k;
+                        ++nn;
+                    }
+                }
+            });
+            listed = nn <= kListMax ? nn : kNoList;
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
             if (nparts > 1u) {
@@ -1159,10 +1256,18 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             });
         }
         if (nparts > 1u) {
-            float v[1] = {acc};
-            if (!combine_parts<1>(pp, wt, part, nparts, v))
-                return;
-            acc = v[0];
+            if constexpr (kCount) {
+                float v[2] = {acc, (float)cnt};  // counts stay far below 2^24: exact
+                if (!combine_parts<2>(pp, wt, part, nparts, v))
+                    return;
+                acc = v[0];
+                cnt = (uint32_t)v[1];
+            } else {
+                float v[1] = {acc};
+                if (!combine_parts<1>(pp, wt, part, nparts, v))
+                    return;
+                acc = v[0];
+            }
         }
         if (!active)
             return;
@@ -1172,7 +1277,9 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         posR[s] = make_float4(pi.x, pi.y, pi.z, 0.5f / r);
         const float4 vs = vel[s];
         velR[s] = make_float4(vs.x, vs.y, vs.z, p);
-        if (MODE >= 1)
+        if (nl.count)
+            nl.count[s] = listed;
+        if (kCount)
             dump.count[s] = cnt;
     });
 }
@@ -1202,30 +1309,36 @@ inline uint32_t stencil_waves(const SceneSet &S, const PartPlan *pp, uint32_t re
 
 template <int MODE, int BLOCK>
 static void launch_density_block(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
-                                 const SceneSet &S, const NeighborDump &d, const PartPlan *pp)
+                                 const SceneSet &S, const NeighborDump &d, const PartPlan *pp,
+                                 const NeighborLists *lists)
 {
     PartPlan p{};
     if (pp)
         p = *pp;
+    NeighborLists nl{nullptr, nullptr, 0u};
+    if (lists)
+        nl = *lists;
     static const uint32_t resident = occupancy_waves(reinterpret_cast<const void *>(&k_density<MODE, BLOCK>), BLOCK);
     hipLaunchKernelGGL((k_density<MODE, BLOCK>), dim3(std::max(1u, blocks_for_waves(stencil_waves(S, pp, resident), BLOCK))),
-                       dim3(BLOCK), 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB, a.posR, a.velR, S, d, p);
+                       dim3(BLOCK), 0, st, a.posB, a.velB, a.cellcB, c.cstart, a.rhoB, a.posR, a.velR, S, d, p, nl);
 }
 
 void launch_density(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                    const NeighborDump *dump, const PartPlan *parts, int block)
+                    const NeighborDump *dump, const PartPlan *parts, int block, const NeighborLists *lists)
 {
     NeighborDump d = dump ? *dump : NeighborDump{nullptr, nullptr, nullptr};
+    if (dump && dump->production)  // the step's own walks, plus the count
+        return launch_density_block<3, kStencilBlock>(st, a, c, S, d, parts, lists);
     if (dump)  // the dump modes walk every lane's whole stencil in one task
-        return dump->out ? launch_density_block<2, kStencilBlock>(st, a, c, S, d, nullptr)
-                         : launch_density_block<1, kStencilBlock>(st, a, c, S, d, nullptr);
+        return dump->out ? launch_density_block<2, kStencilBlock>(st, a, c, S, d, nullptr, nullptr)
+                         : launch_density_block<1, kStencilBlock>(st, a, c, S, d, nullptr, nullptr);
     switch (block) {
     case 64:
-        return launch_density_block<0, 64>(st, a, c, S, d, parts);
+        return launch_density_block<0, 64>(st, a, c, S, d, parts, lists);
     case 256:
-        return launch_density_block<0, 256>(st, a, c, S, d, parts);
+        return launch_density_block<0, 256>(st, a, c, S, d, parts, lists);
     default:
-        return launch_density_block<0, kStencilBlock>(st, a, c, S, d, parts);
+        return launch_density_block<0, kStencilBlock>(st, a, c, S, d, parts, lists);
     }
 }
 
@@ -1314,8 +1427,8 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
     const float4 *__restrict__ posR, const float4 *__restrict__ velR,
     const uint32_t *__restrict__ cellcB, const uint32_t *__restrict__ cstart,
     const float *__restrict__ rhoB, float4 *__restrict__ posA, float4 *__restrict__ velA,
-    uint32_t *__restrict__ cellcA, uint32_t *__restrict__ rankA, uint32_t *__restrict__ counts,
-    float4 *__restrict__ force_out, SceneSet S, PartPlan pp)
+    uint32_t *__restrict__ cellcA, uint32_t *__restrict__ counts,
+    float4 *__restrict__ force_out, SceneSet S, PartPlan pp, NeighborLists nl)
 {
     __shared__ StageBuf<2> stage[BLOCK / 64];
     for_each_part<BLOCK>(S, pp, pp.queue ? pp.queue + 1 : nullptr, [&](uint32_t wt, uint32_t part,
@@ -1336,9 +1449,52 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
         const float mu2_m_lap = 2.0f * P.mu_m_lap;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
 
+        // The term of neighbour j in the order the walks visit it.
+        auto pair = [&](uint32_t j, float4 q, float4 w) {
+            float dx, dy, dz;
+            const float d2 = dist2_ref(pi, q, dx, dy, dz);
+            float am = q.w;  // 0.5 / rho_j
+            if (slow) {
+                int cx = ci.x, cy = ci.y, cz = ci.z;
+                asm volatile("" : "+v"(cx), "+v"(cy), "+v"(cz));
+                am = q.w * (float)bucket_multiplicity(cx, cy, cz, __float_as_uint(posB[j].w) & 0xFFFFu);
+            }
+            force_pair(d2, dx, dy, dz, am, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);
+        };
+        // Every lane holds its neighbour list (the density pass walked this
+        // task per lane, unsplit): visit the listed slots four at a time,
+        // the next group of four slots loaded while the current one runs;
+        // entries past a lane's list stand for its own slot and add nothing.
+        const uint32_t listed = (nl.count && nparts == 1u) ? nl.count[s] : kNoList;
+        const bool use_list = nl.count && nparts == 1u && __all(listed != kNoList);
         UnionRows U;
         uint32_t ucount = 0;
-        if (walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {  // as in k_density
+        if (use_list) {
+            const uint32_t groups = ((uint32_t)wave_max_i((int)listed) + 3u) >> 2;
+            const uint4 *quads = reinterpret_cast<const uint4 *>(nl.list) + s;
+            const size_t stride = nl.stride;
+            // groups <= kListMax / 4, so every group loaded lies inside the
+            // list array; entries a lane has not written are never used
+            uint4 next = quads[0];
+            for (uint32_t g = 0; g < groups; ++g) {
+                const uint32_t k = 4u * g;
+                const uint4 cur = next;
+                if (g + 1u < groups)
+                    next = quads[(size_t)(g + 1u) * stride];
+                const uint32_t j0 = k < listed ? cur.x : s, j1 = k + 1u < listed ? cur.y : s;
+                const uint32_t j2 = k + 2u < listed ? cur.z : s, j3 = k + 3u < listed ? cur.w : s;
+                const float4 q0 = posR[j0], w0 = velR[j0], q1 = posR[j1], w1 = velR[j1];
+                const float4 q2 = posR[j2], w2 = velR[j2], q3 = posR[j3], w3 = velR[j3];
+                if (k < listed)
+                    pair(j0, q0, w0);
+                if (k + 1u < listed)
+                    pair(j1, q1, w1);
+                if (k + 2u < listed)
+                    pair(j2, q2, w2);
+                if (k + 3u < listed)
+                    pair(j3, q3, w3);
+            }
+        } else if (walk_of(pp, wt, cstart, cc, P, slow, U, ucount)) {  // as in k_density
             const float4 *const src[2] = {posR, velR};
             const uint32_t lo = (uint32_t)((uint64_t)ucount * part / nparts);
             const uint32_t hi = (uint32_t)((uint64_t)ucount * (part + 1u) / nparts);
@@ -1373,17 +1529,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
                         const uint32_t k = (uint32_t)__builtin_ctz(mask);
                         mask &= mask - 1u;
                         const uint32_t j = jj + k;
-                        const float4 q = posR[j];  // w = 0.5 / rho_j
-                        const float4 w = velR[j];  // w = p_j
-                        float dx, dy, dz;
-                        const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                        float am = q.w;
-                        if (slow) {
-                            int cx = ci.x, cy = ci.y, cz = ci.z;
-                            asm volatile("" : "+v"(cx), "+v"(cy), "+v"(cz));
-                            am = q.w * (float)bucket_multiplicity(cx, cy, cz, __float_as_uint(posB[j].w) & 0xFFFFu);
-                        }
-                        force_pair(d2, dx, dy, dz, am, w, vi, p_i, h, neg_m_spg, mu2_m_lap, fx, fy, fz);
+                        pair(j, posR[j], velR[j]);  // (x_j, 0.5 / rho_j), (v_j, p_j)
                     }
                 }
             });
@@ -1405,7 +1551,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
             Classified c = classify(pi.x, pi.y, pi.z, P);
             pi.w = __uint_as_float(c.h16);
             cellcA[s] = c.cellc;
-            rankA[s] = atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);
+            atomicAdd(&counts[linear_cell(P, c.cellc)], 1u);  // no result: nothing waits on it
         } else {
             pi.w = 0.0f;
         }
@@ -1416,18 +1562,21 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
 
 template <int BLOCK>
 static void launch_force_block(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
-                               float4 *force_out, bool next_keys, const PartPlan *pp)
+                               float4 *force_out, bool next_keys, const PartPlan *pp, const NeighborLists *lists)
 {
     PartPlan p{};
     if (pp)
         p = *pp;
+    NeighborLists nl{nullptr, nullptr, 0u};
+    if (lists)
+        nl = *lists;
     static const uint32_t resident =
         occupancy_waves(reinterpret_cast<const void *>(&k_force_integrate<false, true, BLOCK>), BLOCK);
     dim3 g(std::max(1u, blocks_for_waves(stencil_waves(S, pp, resident), BLOCK))), b(BLOCK);
 #define SPH_LAUNCH_FI(WF, NK)                                                                          \
     hipLaunchKernelGGL((k_force_integrate<WF, NK, BLOCK>), g, b, 0, st, a.posB, a.velB, a.posR, a.velR, \
-                       a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, a.rankA, c.counts,         \
-                       force_out, S, p)
+                       a.cellcB, c.cstart, a.rhoB, a.posA, a.velA, a.cellcA, c.counts,         \
+                       force_out, S, p, nl)
     if (force_out && next_keys)
         SPH_LAUNCH_FI(true, true);
     else if (force_out)
@@ -1441,15 +1590,15 @@ static void launch_force_block(hipStream_t st, const ParticleArrays &a, const Ce
 
 void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellArrays &c,
                             const SceneSet &S, float4 *force_out, bool next_keys, const PartPlan *parts,
-                            int block)
+                            int block, const NeighborLists *lists)
 {
     switch (block) {
     case 64:
-        return launch_force_block<64>(st, a, c, S, force_out, next_keys, parts);
+        return launch_force_block<64>(st, a, c, S, force_out, next_keys, parts, lists);
     case 256:
-        return launch_force_block<256>(st, a, c, S, force_out, next_keys, parts);
+        return launch_force_block<256>(st, a, c, S, force_out, next_keys, parts, lists);
     default:
-        return launch_force_block<kStencilBlock>(st, a, c, S, force_out, next_keys, parts);
+        return launch_force_block<kStencilBlock>(st, a, c, S, force_out, next_keys, parts, lists);
     }
 }
 

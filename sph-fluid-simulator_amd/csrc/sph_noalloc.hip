@@ -52,6 +52,7 @@ struct Layout {
     size_t force, perm16, counts16, cstart16, partials16;
     size_t params, tasks;
     size_t nparts, off, ppartials, measure, total, ptasks, queue, buf, done, walk, own;
+    size_t nlist, ncount;
     size_t bytes;  // end of the last buffer
     uint32_t ncells_padded, big_cap, nwaves, waves_padded, cap;
 };
@@ -117,6 +118,8 @@ Layout make_layout(uint32_t n, uint64_t ncells)
     L.done = c.take(4ull * L.nwaves);
     L.walk = c.take(2ull * sizeof(int4) * L.nwaves);
     L.own = c.take(4ull * n);
+    L.nlist = c.take(4ull * kListMax * n);
+    L.ncount = c.take(4ull * n);
     L.bytes = c.at;
     return L;
 }
@@ -276,14 +279,15 @@ int update_particles_noalloc(sph_particle_v1 *particles, float *transforms, size
     launch_fill_u32(st, (uint32_t *)pp.total, 0u, 2);
 
     launch_init_keys(st, a, c, set);
-    launch_scan(st, c);
+    launch_scan(st, c, false);
     launch_scatter(st, a, c, set);
     launch_sort_big_cells(st, a, c);
     launch_reorder(st, a, c, set);
     launch_plan_parts(st, a, c, set, pp);
     float4 *force = (float4 *)at(L.force);
-    launch_density(st, a, c, set, nullptr, &pp);
-    launch_force_integrate(st, a, c, set, force, /*next_keys=*/false, &pp);
+    const NeighborLists nl{(uint32_t *)at(L.nlist), (uint32_t *)at(L.ncount), n};
+    launch_density(st, a, c, set, nullptr, &pp, kStencilBlock, &nl);
+    launch_force_integrate(st, a, c, set, force, /*next_keys=*/false, &pp, kStencilBlock, &nl);
 
     CellArrays c16;
     c16.counts = (uint32_t *)at(L.counts16);
@@ -295,7 +299,7 @@ int update_particles_noalloc(sph_particle_v1 *particles, float *transforms, size
     uint32_t *perm16 = (uint32_t *)at(L.perm16);
     launch_fill_u32(st, c16.counts, 0u, 65536);
     launch_h16_rank(st, a, c16, n);
-    launch_scan(st, c16);
+    launch_scan(st, c16, true);
     launch_h16_scatter(st, a, c16, n, perm16);
     void *aos_out = at(L.aos_out);
     float *xf = transforms ? (float *)at(L.xf) : nullptr;

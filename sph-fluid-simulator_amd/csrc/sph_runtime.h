@@ -52,6 +52,8 @@ struct Workspace {
     // walk lengths and their total, part tasks, queue counters, partial sums,
     // finished-part counters
     DevBuf pnparts, poff, ppartials, pmeasure, ptotal, ptasks, pqueue, pbuf, pdone, pwalk, pown;
+    // neighbour lists the density pass writes and the force pass reads
+    DevBuf nlist, ncount;
     uint32_t parts_padded = 0;
     uint32_t ncells_padded = 0;
     uint32_t big_cap = 0;
@@ -68,6 +70,7 @@ struct Workspace {
     CellArrays cells() const;
     SceneSet scene_set() const;
     PartPlan part_plan() const;
+    NeighborLists lists() const;
     void release_all();
 };
 

@@ -253,7 +253,8 @@ int Workspace::reserve_particles(uint32_t n)
         (rc = cellcA.reserve(4ull * n)) || (rc = rankA.reserve(4ull * n)) ||
         (rc = cellcB.reserve(4ull * n)) || (rc = perm.reserve(4ull * n)) ||
         (rc = rhoB.reserve(4ull * n)) || (rc = posR.reserve(sizeof(float4) * (size_t)n)) ||
-        (rc = velR.reserve(sizeof(float4) * (size_t)n)))
+        (rc = velR.reserve(sizeof(float4) * (size_t)n)) || (rc = ncount.reserve(4ull * n)) ||
+        (rc = nlist.reserve(4ull * kListMax * n)))
         return rc;
     return SPH_OK;
 }
@@ -385,11 +386,25 @@ PartPlan Workspace::part_plan() const
     return p;
 }
 
+NeighborLists Workspace::lists() const
+{
+    NeighborLists l;
+    // the slots both arrays cover (they grow together)
+    const size_t slots = std::min<size_t>(ncount.bytes / 4, nlist.bytes / (4ull * kListMax));
+    l.list = (uint32_t *)nlist.ptr;
+    l.count = (uint32_t *)ncount.ptr;
+    l.stride = (uint32_t)slots;
+    if (slots == 0)
+        l.list = l.count = nullptr;
+    return l;
+}
+
 void Workspace::release_all()
 {
     for (DevBuf *b : {&posA, &velA, &posB, &velB, &cellcA, &rankA, &cellcB, &perm, &rhoB, &posR, &velR,
                       &counts, &cstart, &partials, &big, &force, &params, &tasks, &pnparts, &poff,
-                      &ppartials, &pmeasure, &ptotal, &ptasks, &pqueue, &pbuf, &pdone, &pwalk, &pown})
+                      &ppartials, &pmeasure, &ptotal, &ptasks, &pqueue, &pbuf, &pdone, &pwalk, &pown,
+                      &nlist, &ncount})
         b->release();
     ncells_padded = 0;
     big_cap = 0;
@@ -405,7 +420,7 @@ void enqueue_step(hipStream_t st, Workspace &w, float4 *force_out, bool next_key
     SceneSet S = w.scene_set();
     if (timer)
         timer->mark(st, nullptr);
-    launch_scan(st, c);
+    launch_scan(st, c, false);
     if (timer)
         timer->mark(st, "scan");
     launch_scatter(st, a, c, S);
@@ -417,10 +432,11 @@ void enqueue_step(hipStream_t st, Workspace &w, float4 *force_out, bool next_key
     launch_plan_parts(st, a, c, S, pp);
     if (timer)
         timer->mark(st, "reorder");
-    launch_density(st, a, c, S, nullptr, &pp, w.stencil_block);
+    const NeighborLists nl = w.lists();
+    launch_density(st, a, c, S, nullptr, &pp, w.stencil_block, &nl);
     if (timer)
         timer->mark(st, "density");
-    launch_force_integrate(st, a, c, S, force_out, next_keys, &pp, w.stencil_block);
+    launch_force_integrate(st, a, c, S, force_out, next_keys, &pp, w.stencil_block, &nl);
     if (timer)
         timer->mark(st, "force_integrate");
 }
@@ -509,7 +525,7 @@ int stateless_prepare(Stateless &S, const sph_particle_v1 *particles, uint32_t n
     SceneSet set = w.scene_set();
     launch_fill_u32(st, c.counts, 0u, c.ncells_padded);
     launch_init_keys(st, a, c, set);
-    launch_scan(st, c);
+    launch_scan(st, c, false);
     launch_scatter(st, a, c, set);
     launch_sort_big_cells(st, a, c);
     launch_reorder(st, a, c, set);
@@ -576,12 +592,13 @@ int update_particles(sph_particle_v1 *particles, float *transforms, size_t n_,
     const SceneSet set = w.scene_set();
     const PartPlan pp = w.part_plan();
     launch_plan_parts(st, a, c, set, pp);
-    launch_density(st, a, c, set, nullptr, &pp);
-    launch_force_integrate(st, a, c, set, (float4 *)w.force.ptr, /*next_keys=*/false, &pp);
+    const NeighborLists nl = w.lists();
+    launch_density(st, a, c, set, nullptr, &pp, kStencilBlock, &nl);
+    launch_force_integrate(st, a, c, set, (float4 *)w.force.ptr, /*next_keys=*/false, &pp, kStencilBlock, &nl);
     CellArrays c16 = cells16(S);
     launch_fill_u32(st, c16.counts, 0u, 65536);
     launch_h16_rank(st, a, c16, n);
-    launch_scan(st, c16);
+    launch_scan(st, c16, true);
     launch_h16_scatter(st, a, c16, n, (uint32_t *)S.perm16.ptr);
     launch_pack_out(st, a, c16, (const uint32_t *)S.perm16.ptr, (const float4 *)w.force.ptr,
                     S.aos_in.ptr, n, P.xf_scale, S.aos_out.ptr, transforms ? (float *)S.xf.ptr : nullptr);

@@ -12,9 +12,10 @@ namespace sph {
 namespace {
 
 // Device bytes per particle of a Workspace: posA, velA, posB, velB, posR,
-// velR (16 B each), cellcA, rankA, cellcB, perm, rhoB and the part plan's
-// own-stencil length (4 B each).
-constexpr uint64_t kBytesPerParticle = 6 * 16 + 6 * 4;
+// velR (16 B each), cellcA, rankA, cellcB, perm, rhoB, the part plan's
+// own-stencil length and the neighbour-list count (4 B each), and the
+// neighbour list (kListMax x 4 B).
+constexpr uint64_t kBytesPerParticle = 6 * 16 + 7 * 4 + 4 * kListMax;
 // Per 64-slot wave task, the part plan: part count, offset, walk length and
 // done counter (4 B each) and the walk (32 B); per part task (one per wave
 // task plus the scene's part room) its entry (4 B) and partial-sum records

@@ -142,6 +142,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_hip_download": (c_int, [c_void_p] * 6),
         "sph_hip_download_particles": (c_int, [c_void_p, c_void_p, c_void_p]),
         "sph_hip_set_graph": (c_int, [c_void_p, c_int]),
+        "sph_hip_walk_dump": (c_int, [c_void_p] * 5),
+        "sph_batch_walk_dump": (c_int, [c_void_p, c_uint32] + [c_void_p] * 4),
         "sph_hip_count": (c_uint32, [c_void_p]),
         "sph_hip_profile": (
             c_int, [c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_int), c_char_p, c_size_t]),
@@ -357,6 +359,11 @@ class Context:
         _check(fn(self._h, trial_steps, ctypes.byref(t)))
         return t.as_dict()
 
+    def walk_dump(self, lists: bool = False) -> dict:
+        """Neighbour counts and walk classes of the production walks on the
+        current state (sph_hip_walk_dump), by particle id."""
+        return _walk_dump(lambda *a: self._lib.sph_hip_walk_dump(self._h, *a), self.n, lists)
+
     def profile(self, nsteps: int = 10) -> dict:
         ms = (c_double * 32)()
         nk = c_int(0)
@@ -364,6 +371,19 @@ class Context:
         _check(self._lib.sph_hip_profile(self._h, nsteps, ms, 32, ctypes.byref(nk), names, 1024))
         labels = names.raw.split(b"\0")[: nk.value]
         return {labels[k].decode(): float(ms[k]) for k in range(nk.value)}
+
+
+LIST_MAX = 64  # neighbour-list entries per particle (kListMax)
+
+
+def _walk_dump(fn, n: int, lists: bool):
+    count = np.zeros(n, np.uint32)
+    walk = np.zeros(n, np.uint32)
+    lcount = np.zeros(n, np.uint32)
+    lst = np.zeros((n, LIST_MAX), np.uint32) if lists else None
+    _check(fn(_ptr(count), _ptr(walk), _ptr(lcount), _ptr(lst)))
+    return {"count": count, "union": (walk & 1) != 0, "split": (walk & 2) != 0, "listed": (walk & 4) != 0,
+            "list_count": lcount, "list": lst}
 
 
 def scene_desc(n: int, s: Settings, box_: Box, dt: float, cell_scale: float = 1.0,
@@ -443,6 +463,10 @@ class Batch:
         fn = self._lib.sph_batch_autotune_grid if grid else self._lib.sph_batch_autotune
         _check(fn(self._h, trial_steps, ctypes.byref(t)))
         return t.as_dict()
+
+    def walk_dump(self, scene: int, lists: bool = False) -> dict:
+        """As Context.walk_dump, for one scene of the batch."""
+        return _walk_dump(lambda *a: self._lib.sph_batch_walk_dump(self._h, scene, *a), self.counts[scene], lists)
 
     def profile(self, nsteps: int = 10) -> dict:
         ms = (c_double * 32)()

@@ -90,6 +90,10 @@ class Oracle:
                                      c_void_p, c_size_t]),
             "density_subset": (None, [c_void_p, c_size_t, POINTER(c_uint32), POINTER(self.Settings), c_void_p,
                                       c_size_t, c_void_p, c_void_p]),
+            "neighbors_subset": (c_size_t, [c_void_p, c_size_t, POINTER(c_uint32), POINTER(self.Settings),
+                                            c_void_p, c_size_t, c_void_p, c_void_p, c_size_t]),
+            "forces_subset": (None, [c_void_p, c_size_t, POINTER(c_uint32), POINTER(self.Settings), c_void_p,
+                                     c_size_t, c_void_p]),
             "sizeof_particle": (c_int, []),
             "sizeof_settings": (c_int, []),
         }
@@ -192,3 +196,24 @@ class Oracle:
                              which.ctypes.data_as(c_void_p), len(which), rho.ctypes.data_as(c_void_p),
                              cnt.ctypes.data_as(c_void_p))
         return rho, cnt
+
+    def forces_subset(self, p_sorted, table, s, which):
+        """Forces (k x 3) of the selected sorted indices, from the densities
+        and pressures p_sorted holds."""
+        which = np.ascontiguousarray(which, dtype=np.uint32)
+        f = np.zeros((len(which), 3), dtype=self.np_real)
+        self._forces_subset(p_sorted.ctypes.data_as(c_void_p), len(p_sorted), table, ctypes.byref(s),
+                            which.ctypes.data_as(c_void_p), len(which), f.ctypes.data_as(c_void_p))
+        return f
+
+    def neighbors_subset(self, p_sorted, table, s, which):
+        """CSR over the selected sorted indices: their rows of sorted
+        indices, each neighbour repeated m_ij times."""
+        which = np.ascontiguousarray(which, dtype=np.uint32)
+        off = np.zeros(len(which) + 1, dtype=np.uint64)
+        args = (p_sorted.ctypes.data_as(c_void_p), len(p_sorted), table, ctypes.byref(s),
+                which.ctypes.data_as(c_void_p), len(which))
+        total = self._neighbors_subset(*args, off.ctypes.data_as(c_void_p), None, 0)
+        idx = np.zeros(max(total, 1), dtype=np.uint32)
+        self._neighbors_subset(*args, off.ctypes.data_as(c_void_p), idx.ctypes.data_as(c_void_p), len(idx))
+        return off, idx[:total]

@@ -373,3 +373,28 @@ def test_fp64_flavour_agrees_with_fp32(orc32, orc64):
                        orc64.box((-1, -1, -1), (1, 1, 1)))
     np.testing.assert_allclose(a["density"], b["density"], rtol=1e-5)
     np.testing.assert_allclose(a["position"], b["position"], rtol=0, atol=1e-5)
+
+
+def test_subset_functions_equal_the_full_passes(orc32):
+    """neighbors_subset / density_subset / forces_subset (used where only a
+    sample of a large state is checked) give exactly what the full passes
+    give for the same particles."""
+    h = 0.15
+    pos = _axis_cloud(900, h, 31)
+    vel = np.random.default_rng(32).normal(0.0, 0.3, size=pos.shape).astype(F32)
+    s = orc32.settings(0.02, 1000.0, 1.0, 1.04, h, -9.8, 0.2)
+    p = orc32.particles(pos, vel, ids=np.arange(len(pos)))
+    table = orc32.prepare(p, s, nthreads=2)
+    off, idx = orc32.neighbors(p, table, s)
+    orc32.density(p, table, s, nthreads=2)
+    orc32.forces(p, table, s, nthreads=2)
+    which = np.random.default_rng(33).choice(len(pos), size=150, replace=False)
+    so, si = orc32.neighbors_subset(p, table, s, which)
+    rho, cnt = orc32.density_subset(p, table, s, which)
+    f = orc32.forces_subset(p, table, s, which)
+    orc32.free_table(table)
+    for a, i in enumerate(which):
+        assert np.array_equal(si[so[a]:so[a + 1]], idx[off[i]:off[i + 1]])
+        assert cnt[a] == off[i + 1] - off[i]
+    assert np.array_equal(rho, p["density"][which])
+    assert np.array_equal(f, p["force"][which])
