@@ -142,6 +142,45 @@ def load_traffic(kernel):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def load_valu_busy(kernel, window):
+    """VALU-busy fraction of `kernel` from a committed counter summary
+    (profiles/valu_busy_workload2.json: window -> kernel -> fraction), or
+    None."""
+    path = os.path.join(REPO, "profiles", "valu_busy_workload2.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("valu_busy", {}).get(window, {}).get(kernel)
+
+
+def roofline_of(prof, n, value, window):
+    """The roofline object of a per-kernel profile: the dominant kernel's
+    algorithmic bytes per launch over its mean launch time against the HBM
+    peak, beside its VALU-busy fraction (the passes are bound by their
+    arithmetic and load latency, not by HBM)."""
+    dom = max(prof, key=prof.get)
+    ach = ALG_BYTES[dom] * n / (prof[dom] * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "achieved": round(ach, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(ach / HBM_PEAK_GBS, 4),
+        "traffic": load_traffic(dom),
+        "kernel": dom,
+        "kernel_ms": round(prof[dom], 5),
+        "valu_busy": load_valu_busy(dom, window),
+        "alg_bytes_per_particle": ALG_BYTES[dom],
+        "step_alg_GBs": round(ALG_BYTES_STEP * value / 1e9, 2),
+        "step_frac": round(ALG_BYTES_STEP * value / 1e9 / HBM_PEAK_GBS, 4),
+        "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()},
+        # each pass's algorithmic bytes per launch over its mean duration
+        "per_kernel_alg_GBs": {k: round(ALG_BYTES[k] * n / (v * 1e-3) / 1e9, 2) for k, v in prof.items()
+                               if k in ALG_BYTES and v > 0},
+        "l2_hit": load_l2_hit(),
+    }
+
+
 def load_l2_hit():
     """Per-pass L2 hit rates (early and late in the job) from a committed
     counter summary (profiles/l2_hit_workload2.json), or None."""
@@ -178,9 +217,9 @@ def dropin_leg(sc, calls=5):
             "note": "stateless sph_hip_update_particles_box on host AoS + transforms, steps 1-6 of the job"}
 
 
-def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024):
+def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024, auto_grid=True):
     scenes = sph_scenes.batched_workload(nscenes, seed)
-    descs = [sph_serving.scene_desc(sc) for sc in scenes]
+    descs = [sph_serving.scene_desc(sc, auto_grid=auto_grid) for sc in scenes]
     total = sum(sc.n for sc in scenes)
     log(f"batched: {nscenes} scenes, {total} particles, {steps} steps")
     # The grid tuner (block size x cell scale 1 / 0.5) on the initial states, in
@@ -210,7 +249,7 @@ def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024):
     # the same scenes, one after another, each on its own single-scene context
     seq = 0.0
     for sc, d in zip(scenes, descs):
-        with sph_amd.Context(sc.n, d.settings, d.box, sc.dt) as ctx:
+        with sph_amd.Context(sc.n, d.settings, d.box, sc.dt, auto_grid=auto_grid) as ctx:
             ctx.upload(sc.pos, sc.vel)
             ctx.step(1)  # graph capture
             ctx.upload(sc.pos, sc.vel)
@@ -223,7 +262,8 @@ def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024):
     log(f"sequential single-scene stepping: {seq:.3f} s, {seq_rate:.0f} scene-steps/s")
     return {
         "workload": f"{nscenes} scenes, particles log-uniform in [1k, 64k] (total {total}), h = 2.1 spacing, "
-                    f"viscosity U[0.5,2], gas constant U[0.5,3], {steps} steps, one launch per pass, graphed",
+                    f"viscosity U[0.5,2], gas constant U[0.5,3], {steps} steps, one launch per pass, graphed, "
+                    f"cell size {'automatic per scene' if auto_grid else 'h'}",
         "scene_steps_per_s": batched_rate,
         "particle_updates_per_s": total * steps / wall,
         "step_ms_p50": float(np.percentile(ms, 50)),
@@ -232,7 +272,8 @@ def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024):
         "speedup_vs_sequential": batched_rate / seq_rate,
         "autotune": tune,
         "autotune_grid": {"initial_states": grid_initial, "end_of_job": grid_final,
-                          "note": "block size x cell scale {1, 0.5}, reported only; the timed job runs at cell scale 1"},
+                          "note": "block size x cell scale {1, 0.5} for all scenes at once, reported only; "
+                                  "the timed job chooses each scene's cell size from its own state"},
     }
 
 
@@ -275,6 +316,10 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying a HIP graph (same kernels; "
                          "for kernel tracers that do not follow graph replays)")
+    ap.add_argument("--fixed-grid", action="store_true",
+                    help="keep cells of h for the whole job instead of the automatic cell size")
+    ap.add_argument("--no-job", action="store_true",
+                    help="skip the whole 1000-step job that is timed beside a shorter window")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -285,7 +330,8 @@ def main():
     log(f"workload {sc.name}: n={sc.n} h={sc.h} dt={sc.dt}")
     s = sph_amd.settings(*sc.settings_args())
     b = sph_amd.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset)
-    ctx = sph_amd.Context(sc.n, s, b, sc.dt)
+    auto = not args.fixed_grid
+    ctx = sph_amd.Context(sc.n, s, b, sc.dt, auto_grid=auto)
     ctx.set_graph(not args.no_graph)
     ctx.upload(sc.pos, sc.vel)
     ctx.step(args.warmup)
@@ -301,31 +347,31 @@ def main():
     value = sc.n * args.steps / elapsed
     log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value / 1e9:.3f} G particle-updates/s")
 
+    window = "early" if args.steps <= 100 else "job"
     roofline = None
     if not args.no_profile:
         ctx.upload(sc.pos, sc.vel)  # same job again, with events around every kernel
         prof = ctx.profile(args.steps)
         log("per-kernel mean ms (HIP events): " + ", ".join(f"{k}={v:.4f}" for k, v in prof.items()))
-        dom = max(prof, key=prof.get)
-        ach = ALG_BYTES[dom] * sc.n / (prof[dom] * 1e-3) / 1e9
-        roofline = {
-            "bound": "hbm",
-            "achieved": round(ach, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(dom),
-            "kernel": dom,
-            "kernel_ms": round(prof[dom], 5),
-            "alg_bytes_per_particle": ALG_BYTES[dom],
-            "step_alg_GBs": round(ALG_BYTES_STEP * value / 1e9, 2),
-            "step_frac": round(ALG_BYTES_STEP * value / 1e9 / HBM_PEAK_GBS, 4),
-            "per_kernel_ms": {k: round(v, 5) for k, v in prof.items()},
-            # each pass's algorithmic bytes per launch over its mean duration
-            "per_kernel_alg_GBs": {k: round(ALG_BYTES[k] * sc.n / (v * 1e-3) / 1e9, 2) for k, v in prof.items()
-                                   if k in ALG_BYTES and v > 0},
-            "l2_hit": load_l2_hit(),
-        }
+        roofline = roofline_of(prof, sc.n, value, window)
+
+    job = None
+    if not args.no_job and args.steps != 1000 and not args.no_reset:
+        # the representative figure: the whole 1000-step job from the initial
+        # state (lattice, collapse, clumps), beside the short window above
+        ctx.upload(sc.pos, sc.vel)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(1000)
+        ctx.sync()
+        jt = time.perf_counter() - t0
+        log(f"whole job: 1000 steps in {jt:.3f} s, {jt:.4f} ms/step, {sc.n * 1000 / jt / 1e9:.3f} G particle-updates/s")
+        job = {"steps": 1000, "ms_per_step": jt, "value": sc.n * 1000 / jt, "unit": "particle-updates/s"}
+        if not args.no_profile:
+            ctx.upload(sc.pos, sc.vel)
+            jprof = ctx.profile(1000)
+            log("whole job per-kernel mean ms: " + ", ".join(f"{k}={v:.4f}" for k, v in jprof.items()))
+            job["roofline"] = roofline_of(jprof, sc.n, job["value"], "job")
     ctx.close()
 
     # measured HBM bandwidth of a plain copy kernel (read + write), beside the 8 TB/s peak
@@ -343,7 +389,7 @@ def main():
         log(f"cpu baseline: {cpu['value'] / 1e6:.3f} M particle-updates/s")
 
     dropin = dropin_leg(sc)
-    batched = None if args.no_batched else batched_leg(steps=args.batched_steps)
+    batched = None if args.no_batched else batched_leg(steps=args.batched_steps, auto_grid=auto)
     serving = None
     if not args.no_serving:
         rates = [float(x) for x in args.serving_rates.split(",") if x]
@@ -367,8 +413,10 @@ def main():
                         f"dt={sc.dt}, box [-2,2]x[-1,1]^2, steps 1..{args.steps} of the job, resident in HBM",
             "particles": sc.n,
             "parallelism": "single GPU",
+            "cell_size": "automatic (h/2 or h, chosen every 64 steps)" if auto else "h",
         },
         "roofline": roofline,
+        "job": job,
         "cpu_baseline": cpu,
         "dropin_pcie_inclusive": dropin,
         "batched": batched,

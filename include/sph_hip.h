@@ -96,6 +96,16 @@ typedef struct sph_scene_desc_v1 {
  * differences).  The flag exists for tests. */
 #define SPH_SCENE_LANE_WALK 2u
 
+/* The cell size follows the state: before the scene's first step and every
+ * 64 steps after that, the candidates per neighbour of a sample of its
+ * particles choose cells of h / 2 (a state uniform at the scale of h: about
+ * 40% fewer candidates per particle) or of h (a state clumped below h,
+ * where finer cells only add stencil rows).  Scenes of fewer than 131,072
+ * particles keep cells of h.  The choice depends only on the scene's own
+ * state and step count, so the scene steps the same alone and in any batch;
+ * results move by rounding when the grid changes. */
+#define SPH_SCENE_AUTO_GRID 4u
+
 /* ---- host-only helpers (no device work) --------------------------------- */
 
 int sph_abi_version(void);
@@ -199,6 +209,11 @@ int sph_hip_download_particles(sph_ctx *ctx, sph_particle_v1 *out,
  * entries per id, the listed neighbours' ids (each once) in walk order. */
 int sph_hip_walk_dump(sph_ctx *ctx, uint32_t *count, uint32_t *walk,
                       uint32_t *list_count, uint32_t *list);
+/* The scene's current cell scale (in units of h (1 + 2^-10)) and the
+ * candidates per neighbour of a sample of its particles in the last step's
+ * state, computed now on the current grid (0 before the first step): the
+ * statistic SPH_SCENE_AUTO_GRID decides by.  Either pointer may be NULL. */
+int sph_hip_grid_info(sph_ctx *ctx, float *cell_scale, double *cand_per_nbr);
 /* 1 = replay steps through a captured HIP graph (default), 0 = launch. */
 int sph_hip_set_graph(sph_ctx *ctx, int enable);
 uint32_t sph_hip_count(const sph_ctx *ctx);
@@ -222,6 +237,13 @@ int sph_batch_sync(sph_batch *b);
 int sph_batch_download(sph_batch *b, uint32_t scene, float *pos, float *vel,
                        float *rho, float *pressure, uint32_t *ids);
 int sph_batch_set_graph(sph_batch *b, int enable);
+/* sph_hip_download_particles for one scene of a batch (forces need
+ * SPH_SCENE_KEEP_FORCE on some scene of the batch). */
+int sph_batch_download_particles(sph_batch *b, uint32_t scene,
+                                 sph_particle_v1 *out, float *transforms);
+/* sph_hip_grid_info for one scene of a batch. */
+int sph_batch_grid_info(sph_batch *b, uint32_t scene, float *cell_scale,
+                        double *cand_per_nbr);
 /* sph_hip_walk_dump for one scene of a batch. */
 int sph_batch_walk_dump(sph_batch *b, uint32_t scene, uint32_t *count,
                         uint32_t *walk, uint32_t *list_count, uint32_t *list);

@@ -11,11 +11,32 @@ namespace sph {
 
 constexpr uint32_t kMaxParticles = 1u << 30;
 
+// Automatic cell size (SPH_SCENE_AUTO_GRID): before a scene's first step
+// and every kRegridEvery steps after that, the sampled candidates per
+// neighbour of its current state choose its cell scale: a state uniform at
+// the scale of h takes cells of h / 2 (about 40% fewer candidates), one
+// clumped below h takes cells of h (fewer stencil rows).  The choice
+// depends on the scene's state and step count alone.
+constexpr uint64_t kRegridEvery = 64;
+// Thresholds measured on the 1M-particle dam break: cells of h / 2 run its
+// steps faster while the ratio stays above ~6.5 at scale 1 (~3.8 at 1/2;
+// the jittered lattice start reads 7.2 and 4.2), slower after; smaller
+// lattices read less (their surfaces), an 8,000-particle cube 6.3.
+constexpr double kToFineRatio = 6.0;    // at cell scale 1: above this, go to 1/2
+constexpr double kToCoarseRatio = 3.5;  // at cell scale 1/2: below this, go to 1
+// Smaller scenes keep cells of h: on the batched workload (256 scenes of
+// 1k-64k particles) cells of h / 2 made the 500-step job 26% slower (eight
+// times the cells to scan, and no faster density pass at these sizes).
+constexpr uint32_t kAutoGridMinParticles = 1u << 17;
+
 struct BatchScene {
     sph_scene_desc_v1 desc;
     StepParams P;  // constants, grid, slot range
     double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};  // bounds the grid covers
     bool uploaded = false;
+    uint64_t steps = 0;  // steps since the upload
+    double scale = 0.0;  // cell scale of the current grid (0: desc.cell_scale)
+    double last_ratio = 0.0;  // candidates per neighbour at the last evaluation
 };
 
 int validate_desc(const sph_scene_desc_v1 &d);
@@ -38,6 +59,9 @@ struct Batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     DevBuf aos, xf;  // download staging
+    DevBuf gstat;    // grid statistic: candidates, neighbours per scene
+    bool stat_ok = false;    // B order and cell ranges are those of the last step
+    bool hold_grid = false;  // steps neither count nor re-grid (autotune trials)
     std::vector<hipEvent_t> events;
     // Tests only: a failure the next relayout (kFailRelayout) or upload
     // (kFailUpload) reports at its last fallible point, once.
@@ -64,6 +88,17 @@ struct Batch {
     int prepare();
     void enqueue(StepTimer *timer);
     int step(int nsteps);
+    // nsteps steps on the current layout (no grid decisions)
+    int step_plain(int nsteps);
+    // Sampled candidates per neighbour of every scene (ratio[s]): fresh =
+    // from the cell order of the current state (sorted here), else from
+    // the last step's (0 if there is none).
+    int grid_stat(std::vector<double> &ratio, bool fresh);
+    // Evaluates the automatic grid of the scenes whose step count is at a
+    // re-grid point (a multiple of kRegridEvery, the upload included) and
+    // re-grids those whose choice changed.
+    int auto_grid();
+    int grid_info(uint32_t s, float *cell_scale, double *cand_per_nbr);
     // step() with an event after every step; ms[k] (may be NULL) receives
     // the GPU time of step k.  Synchronises.
     int step_timed(int nsteps, float *ms);

@@ -22,7 +22,7 @@
 namespace sph {
 
 namespace {
-constexpr uint32_t kKnownFlags = SPH_SCENE_KEEP_FORCE | SPH_SCENE_LANE_WALK;
+constexpr uint32_t kKnownFlags = SPH_SCENE_KEEP_FORCE | SPH_SCENE_LANE_WALK | SPH_SCENE_AUTO_GRID;
 }
 
 int validate_desc(const sph_scene_desc_v1 &d)
@@ -244,6 +244,7 @@ int Batch::relayout(std::vector<BatchScene> next, const std::vector<int> &from)
     total_n = (uint32_t)total;
     layout_dirty = true;
     keys_dirty = true;
+    stat_ok = false;
     return SPH_OK;
 }
 
@@ -288,8 +289,26 @@ int Batch::upload(uint32_t s, const float *pos, const float *vel)
     if (e != hipSuccess)
         return hip_fail(e, "upload");
     b.uploaded = true;
+    b.steps = 0;
+    b.scale = 0.0;
+    b.last_ratio = 0.0;
     layout_dirty = true;
     keys_dirty = true;
+    stat_ok = false;
+    // Once every scene holds particles, the first step's set-up (cells,
+    // scene table, keys, the automatic grid's first choice) runs here
+    // rather than inside the first step.
+    for (const BatchScene &o : scenes)
+        if (!o.uploaded)
+            return SPH_OK;
+    int rc;
+    if (!hold_grid && (rc = auto_grid()))
+        return rc;
+    if ((rc = prepare()))
+        return rc;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "upload");
     return SPH_OK;
 }
 
@@ -310,6 +329,7 @@ int Batch::prepare()
             table.push_back(b.P);
         }
         drop_graph();
+        stat_ok = false;
         if ((rc = w.reserve_cells(base)) || (rc = w.set_scenes(table, stream)))
             return rc;
         layout_dirty = false;
@@ -337,7 +357,134 @@ int Batch::step(int nsteps)
     int rc;
     if (nsteps < 0)
         return fail(SPH_E_INVALID, "step: negative step count");
-    if (scenes.empty())
+    if (scenes.empty() || nsteps == 0)
+        return SPH_OK;
+    bool any_auto = false;
+    for (const BatchScene &b : scenes)
+        any_auto = any_auto || (b.desc.flags & SPH_SCENE_AUTO_GRID) != 0u;
+    if (hold_grid || !any_auto) {
+        if ((rc = step_plain(nsteps)))
+            return rc;
+        if (!hold_grid)
+            for (BatchScene &b : scenes)
+                b.steps += (uint64_t)nsteps;
+        return SPH_OK;
+    }
+    // Evaluates the automatic scenes due (step count a multiple of
+    // kRegridEvery, the upload included) and runs up to the next point.
+    uint64_t left = (uint64_t)nsteps;
+    while (left > 0) {
+        if ((rc = auto_grid()))
+            return rc;
+        uint64_t k = left;
+        for (const BatchScene &b : scenes)
+            if (b.desc.flags & SPH_SCENE_AUTO_GRID)
+                k = std::min<uint64_t>(k, kRegridEvery - b.steps % kRegridEvery);
+        if ((rc = step_plain((int)k)))
+            return rc;
+        for (BatchScene &b : scenes)
+            b.steps += k;
+        left -= k;
+    }
+    return SPH_OK;
+}
+
+int Batch::grid_stat(std::vector<double> &ratio, bool fresh)
+{
+    int rc;
+    ratio.assign(scenes.size(), 0.0);
+    if (fresh) {
+        // the cell order of the current state (the next step redoes it and
+        // recomputes the keys first: the scatter consumes the cell counts)
+        if ((rc = prepare()))
+            return rc;
+        const ParticleArrays a = w.particles();
+        const CellArrays c = w.cells();
+        const SceneSet S = w.scene_set();
+        launch_scan(stream, c, false);
+        launch_scatter(stream, a, c, S);
+        launch_sort_big_cells(stream, a, c);
+        launch_reorder(stream, a, c, S);
+        keys_dirty = true;
+        stat_ok = true;
+    }
+    if (!stat_ok)
+        return SPH_OK;
+    const size_t ns = scenes.size();
+    if ((rc = gstat.reserve(16ull * ns)))
+        return rc;
+    launch_grid_stat(stream, w.particles(), w.cells(), w.scene_set(), (unsigned long long *)gstat.ptr, (uint32_t)ns);
+    std::vector<unsigned long long> st(2 * ns);
+    hipError_t e = hipMemcpyAsync(st.data(), gstat.ptr, 16ull * ns, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    if (e == hipSuccess)
+        e = hipGetLastError();
+    if (e != hipSuccess)
+        return hip_fail(e, "grid statistic");
+    for (size_t k = 0; k < ns; ++k)
+        ratio[k] = st[2 * k + 1] ? (double)st[2 * k] / (double)st[2 * k + 1] : (st[2 * k] ? INFINITY : 0.0);
+    return SPH_OK;
+}
+
+int Batch::grid_info(uint32_t s, float *cell_scale, double *cand_per_nbr)
+{
+    if (s >= scenes.size())
+        return fail(SPH_E_INVALID, "grid_info: bad scene");
+    const BatchScene &b = scenes[s];
+    if (cell_scale)
+        *cell_scale = (float)(b.scale > 0.0 ? b.scale : (b.desc.cell_scale > 0.0f ? (double)b.desc.cell_scale : 1.0));
+    if (cand_per_nbr) {
+        std::vector<double> ratio;
+        int rc = grid_stat(ratio, false);
+        if (rc)
+            return rc;
+        *cand_per_nbr = ratio[s];
+    }
+    return SPH_OK;
+}
+
+int Batch::auto_grid()
+{
+    int rc;
+    auto due = [](const BatchScene &b) {
+        return (b.desc.flags & SPH_SCENE_AUTO_GRID) != 0u && b.steps % kRegridEvery == 0;
+    };
+    bool any = false;
+    for (const BatchScene &b : scenes)
+        any = any || due(b);
+    if (!any)
+        return SPH_OK;
+    std::vector<double> ratio;
+    if ((rc = grid_stat(ratio, /*fresh=*/true)))
+        return rc;
+    for (size_t k = 0; k < scenes.size(); ++k) {
+        BatchScene &b = scenes[k];
+        if (!due(b))
+            continue;
+        b.last_ratio = ratio[k];
+        const double cur = b.scale > 0.0 ? b.scale : (b.desc.cell_scale > 0.0f ? (double)b.desc.cell_scale : 1.0);
+        double want = cur;
+        if (cur >= 1.0 && ratio[k] > kToFineRatio && b.desc.n >= kAutoGridMinParticles)
+            want = 0.5;
+        else if (cur < 1.0 && ratio[k] < kToCoarseRatio)
+            want = 1.0;
+        if (want != cur) {
+            if ((rc = scene_regrid(b, want)))
+                return rc;
+            b.scale = want;
+            layout_dirty = true;  // new cells, scene table and keys before the next step
+        }
+    }
+    return SPH_OK;
+}
+
+int Batch::step_plain(int nsteps)
+{
+    int rc;
+    if (nsteps < 0)
+        return fail(SPH_E_INVALID, "step: negative step count");
+    if (scenes.empty() || nsteps == 0)
         return SPH_OK;
     if ((rc = prepare()))
         return rc;
@@ -367,6 +514,7 @@ int Batch::step(int nsteps)
         if (e != hipSuccess)
             return hip_fail(e, "step launch");
     }
+    stat_ok = true;  // the B order and cell ranges are the last step's
     return SPH_OK;
 }
 
@@ -430,14 +578,22 @@ int Batch::autotune(int trial_steps, sph_tune_v1 *out, bool grid)
         if (err != hipSuccess)
             return hip_fail(err, "autotune restore");
         keys_dirty = true;
+        stat_ok = false;
         return SPH_OK;
     };
+    // Trial steps neither count toward the scenes' step counts nor re-grid.
+    struct Hold {
+        bool &flag;
+        ~Hold() { flag = false; }
+    } hold{hold_grid};
+    hold_grid = true;
     // The walk (per lane or union) is not a knob: it may change split sums.
     auto apply = [&](int block, double scale) -> int {
         w.stencil_block = block;
         if (grid)
             for (BatchScene &b : scenes) {
                 b.desc.cell_scale = (float)scale;  // kept by later uploads
+                b.scale = 0.0;
                 int r = scene_regrid(b, 0.0);
                 if (r)
                     return r;
@@ -576,8 +732,15 @@ int Batch::profile(int nsteps, double *ms, int max_k, int *nk, char *names_buf, 
     std::vector<double> acc;
     std::vector<const char *> names;
     for (int s = 0; s < nsteps; ++s) {
+        // the same schedule of grid choices as step() (outside the timed kernels)
+        if (!hold_grid && ((rc = auto_grid()) || (rc = prepare())))
+            return rc;
         StepTimer t;
         enqueue(&t);
+        stat_ok = true;
+        if (!hold_grid)
+            for (BatchScene &b : scenes)
+                ++b.steps;
         hipError_t e = hipStreamSynchronize(stream);
         if (e != hipSuccess)
             return hip_fail(e, "profile step");
@@ -636,6 +799,7 @@ int Batch::walk_dump(uint32_t s, uint32_t *count, uint32_t *walk, uint32_t *list
     const NeighborLists nl = w.lists();
     launch_scan(stream, c, false);
     keys_dirty = true;  // the scan consumed the cell counts: the next step recomputes the keys
+    stat_ok = false;    // and the cell ranges now belong to this dump
     launch_scatter(stream, a, c, S);
     launch_sort_big_cells(stream, a, c);
     launch_reorder(stream, a, c, S);
@@ -802,6 +966,27 @@ int sph_hip_walk_dump(sph_ctx *c, uint32_t *count, uint32_t *walk, uint32_t *lis
     if (!c)
         return fail(SPH_E_INVALID, "walk_dump: NULL context");
     return c->b.walk_dump(0, count, walk, list_count, list);
+}
+
+int sph_batch_download_particles(sph_batch *b, uint32_t scene, sph_particle_v1 *out, float *transforms)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "download_particles: NULL batch");
+    return b->b.download_particles(scene, out, transforms);
+}
+
+int sph_hip_grid_info(sph_ctx *c, float *cell_scale, double *cand_per_nbr)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "grid_info: NULL context");
+    return c->b.grid_info(0, cell_scale, cand_per_nbr);
+}
+
+int sph_batch_grid_info(sph_batch *b, uint32_t scene, float *cell_scale, double *cand_per_nbr)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "grid_info: NULL batch");
+    return b->b.grid_info(scene, cell_scale, cand_per_nbr);
 }
 
 int sph_batch_walk_dump(sph_batch *b, uint32_t scene, uint32_t *count, uint32_t *walk, uint32_t *list_count,

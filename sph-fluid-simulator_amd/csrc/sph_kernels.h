@@ -185,6 +185,13 @@ void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellA
                             const SceneSet &S, float4 *force_out, bool next_keys,
                             const PartPlan *parts = nullptr, int block = kStencilBlock,
                             const NeighborLists *lists = nullptr);
+// Sampled candidates per neighbour of every scene (one wave task in
+// kGridStatEvery of each scene), from the current B order and cell ranges:
+// stat[2 s] = candidates, stat[2 s + 1] = neighbours of scene s (cleared
+// first).
+constexpr uint32_t kGridStatEvery = 16;
+void launch_grid_stat(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
+                      unsigned long long *stat, uint32_t nscenes);
 // Drop-in output (single scene): order slots by the pre-step 16-bit bucket
 // (ties by id), and write the reference AoS layout plus transforms.
 void launch_h16_rank(hipStream_t st, const ParticleArrays &a, const CellArrays &c16, uint32_t n);

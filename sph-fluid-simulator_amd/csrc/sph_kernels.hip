@@ -613,6 +613,7 @@ __device__ inline RefCell ref_cell(float4 p, const StepParams &P)
 // cell cc (R = P.reach), clipped to the grid: for row (y, z) the x-run of
 // cells [gx - R, gx + R] is one contiguous range [b, e) of the cell-ordered
 // arrays.  Calls row(b, e) for each row, z-major then y, both ascending.
+// The bounds of the next row are loaded while the current row runs.
 template <class Row>
 __device__ inline void for_each_row(const uint32_t *__restrict__ cstart, uint32_t cc,
                                     const StepParams &P, Row row)
@@ -621,11 +622,28 @@ __device__ inline void for_each_row(const uint32_t *__restrict__ cstart, uint32_
     const int x0 = max(gx - R, 0), x1 = min(gx + R, P.nx - 1);
     const int y0 = max(gy - R, 0), y1 = min(gy + R, P.ny - 1);
     const int z0 = max(gz - R, 0), z1 = min(gz + R, P.nz - 1);
-    for (int z = z0; z <= z1; ++z)
-        for (int y = y0; y <= y1; ++y) {
-            const uint32_t r0 = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
-            row(cstart[r0 + (uint32_t)x0], cstart[r0 + (uint32_t)x1 + 1u]);
+    const int nrows = (y1 - y0 + 1) * (z1 - z0 + 1);
+    auto base = [&](int y, int z) {
+        return P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
+    };
+    int y = y0, z = z0;
+    uint32_t r0 = base(y, z);
+    uint32_t b = cstart[r0 + (uint32_t)x0], e = cstart[r0 + (uint32_t)x1 + 1u];
+    for (int k = 0; k < nrows; ++k) {
+        if (++y > y1) {
+            y = y0;
+            ++z;
         }
+        uint32_t bn = 0, en = 0;
+        if (k + 1 < nrows) {
+            r0 = base(y, z);
+            bn = cstart[r0 + (uint32_t)x0];
+            en = cstart[r0 + (uint32_t)x1 + 1u];
+        }
+        row(b, e);
+        b = bn;
+        e = en;
+    }
 }
 
 // for_each_row restricted to the positions [lo, hi) of the lane's own
@@ -1105,12 +1123,16 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
         Lane L;
         bool active;
         wave_task(S, wt, L, active);
-        const StepParams &P = *L.P;
+        // a copy: read once (scalar loads) before the task's first store
+        const StepParams P = *L.P;
         const uint32_t s = L.slot;
         const float4 pi = pos[s];
         const uint32_t cc = cellc[s];
         const bool slow = (cc & kSlowBit) != 0u;
-        const RefCell ci = ref_cell(pi, P);
+        // the reference cell (three IEEE divisions) only matters for m_ij
+        RefCell ci{0, 0, 0};
+        if (slow)
+            ci = ref_cell(pi, P);
         const float h2 = P.h2, mp6 = P.mp6;
         float acc = 0.0f;
         uint32_t cnt = 0;
@@ -1191,28 +1213,43 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             auto fast = [&](uint32_t j, float4 q, uint32_t g) -> uint64_t {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
-                float t = fmaxf(h2 - d2, 0.0f);
-                t = (j != s) ? t : 0.0f;
+                const float t = fmaxf(h2 - d2, 0.0f);
                 acc = acc + mp6 * ((t * t) * t);
                 if (kCount)
                     cnt += t > 0.0f ? 1u : 0u;
                 return t > 0.0f ? (1ull << (j - g)) : 0ull;
             };
+            // candidates [j0, j1) of the segment starting at g
+            auto run_fast = [&](uint32_t j0, uint32_t j1, uint32_t g, uint64_t &mask) {
+                uint32_t j = j0;
+                for (; j + 4u <= j1; j += 4u) {  // four candidates in flight
+                    const float4 q0 = pos[j], q1 = pos[j + 1u], q2 = pos[j + 2u], q3 = pos[j + 3u];
+                    mask |= fast(j, q0, g);
+                    mask |= fast(j + 1u, q1, g);
+                    mask |= fast(j + 2u, q2, g);
+                    mask |= fast(j + 3u, q3, g);
+                }
+                for (; j < j1; ++j)
+                    mask |= fast(j, pos[j], g);
+            };
+            // Once a lane has more neighbours than a list holds, the force
+            // pass walks this task anyway: the wave stops recording (and
+            // only sums) for the rest of its rows.
+            bool listing = true;
             for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
                 for (uint32_t g = b; g < e; g += 64u) {
                     const uint32_t ge = min(g + 64u, e);
                     uint64_t mask = 0;
                     uint32_t j = g;
                     if (!any_slow) {
-                        for (; j + 4u <= ge; j += 4u) {  // four candidates in flight
-                            const float4 q0 = pos[j], q1 = pos[j + 1u], q2 = pos[j + 2u], q3 = pos[j + 3u];
-                            mask |= fast(j, q0, g);
-                            mask |= fast(j + 1u, q1, g);
-                            mask |= fast(j + 2u, q2, g);
-                            mask |= fast(j + 3u, q3, g);
-                        }
-                        for (; j < ge; ++j)
-                            mask |= fast(j, pos[j], g);
+                        // the segment split at the lane's own slot (when it
+                        // holds it), so no candidate needs a self test
+                        const uint32_t m = (s >= g && s < ge) ? s : ge;
+                        run_fast(g, m, g, mask);
+                        run_fast(min(m + 1u, ge), ge, g, mask);
+                        j = ge;
+                        if (!listing)
+                            continue;
                     }
                     for (; j + 1u < ge; j += 2u) {  // two candidates in flight
                         const float4 q0 = pos[j];
@@ -1224,6 +1261,8 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                     }
                     if (j < ge && j != s && body(j, pos[j]))
                         mask |= 1ull << (j - g);
+                    if (!listing)
+                        continue;
                     while (mask != 0ull) {
                         const uint32_t k = (uint32_t)__builtin_ctzll(mask);
                         mask &= mask - 1ull;
@@ -1231,9 +1270,10 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                             nl.list[list_at(nn, s, nl.stride)] = g + k;
                         ++nn;
                     }
+                    listing = !__any(nn > kListMax);
                 }
             });
-            listed = nn <= kListMax ? nn : kNoList;
+            listed = listing ? nn : kNoList;
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
             if (nparts > 1u) {
@@ -1443,7 +1483,10 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
         float4 vi = make_float4(vr.x, vr.y, vr.z, velB[s].w);  // w = particle id
         const uint32_t cc = cellcB[s];
         const bool slow = (cc & kSlowBit) != 0u;
-        const RefCell ci = ref_cell(pi, P);
+        // the reference cell (three IEEE divisions) only matters for m_ij
+        RefCell ci{0, 0, 0};
+        if (slow)
+            ci = ref_cell(pi, P);
         const float p_i = vr.w;
         const float h = P.h, h2 = P.h2, neg_m_spg = P.neg_m_spg;
         const float mu2_m_lap = 2.0f * P.mu_m_lap;
@@ -1600,6 +1643,65 @@ void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellA
     default:
         return launch_force_block<kStencilBlock>(st, a, c, S, force_out, next_keys, parts, lists);
     }
+}
+
+// ------------------------------------------------------- grid statistic
+// Candidates per neighbour of a sample of every scene: its wave tasks
+// 0, kGridStatEvery, 2 kGridStatEvery, ... (counted from the scene's own
+// first slot, so the sample is the same in any batch) walk their lanes' whole
+// stencils over the current cell ranges and B order, counting the
+// candidates and those within h (the particle itself excluded); each
+// wave adds its sums to stat[2 scene] and stat[2 scene + 1].  The ratio
+// tells a state uniform at the scale of h (a jittered lattice of spacing
+// h / 2 reads 7.2 at cell scale 1, 4.2 at cell scale 1/2) from one clumped
+// below it (near 1).
+__global__ __launch_bounds__(kBlock) void k_grid_stat(const float4 *__restrict__ pos,
+                                                     const uint32_t *__restrict__ cellc,
+                                                     const uint32_t *__restrict__ cstart, SceneSet S,
+                                                     unsigned long long *__restrict__ stat)
+{
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+    if (w >= S.nwaves)
+        return;
+    Lane L;
+    bool active;
+    wave_task(S, w, L, active);
+    const StepParams &P = *L.P;
+    if (((L.first - P.p_begin) / 64u) % kGridStatEvery != 0u)
+        return;  // the sample counts tasks from the scene's own first slot
+    const uint32_t s = L.slot;
+    uint32_t cand = 0, nbr = 0;
+    if (active) {
+        const float4 pi = pos[s];
+        const float h2 = P.h2;
+        for_each_row(cstart, cellc[s], P, [&](uint32_t b, uint32_t e) {
+            cand += e - b;
+            for (uint32_t j = b; j < e; ++j) {
+                float dx, dy, dz;
+                nbr += (j != s && dist2_ref(pi, pos[j], dx, dy, dz) < h2) ? 1u : 0u;
+            }
+        });
+    }
+    unsigned long long c = cand, nb = nbr;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        c += __shfl_xor(c, d, 64);
+        nb += __shfl_xor(nb, d, 64);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        const uint32_t scene = S.tasks[w].scene;
+        atomicAdd(&stat[2u * scene], c);
+        atomicAdd(&stat[2u * scene + 1u], nb);
+    }
+}
+
+void launch_grid_stat(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
+                      unsigned long long *stat, uint32_t nscenes)
+{
+    launch_fill_u32(st, (uint32_t *)stat, 0u, 4ull * nscenes);
+    if (S.nwaves)
+        hipLaunchKernelGGL(k_grid_stat, dim3(blocks_for_waves(S.nwaves, kBlock)), dim3(kBlock), 0, st, a.posB,
+                           a.cellcB, c.cstart, S, stat);
 }
 
 // ------------------------------------------------------- drop-in output

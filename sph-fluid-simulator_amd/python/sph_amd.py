@@ -29,6 +29,7 @@ SPH_E_NOMEM = -3
 SPH_E_HIP = -4
 SPH_E_LIMIT = -5
 SPH_E_STATE = -6
+SPH_SCENE_AUTO_GRID = 4
 SPH_SERVER_FAIL_RELAYOUT = 1
 SPH_SERVER_FAIL_UPLOAD = 2
 NOALLOC_MAX_PARTICLES = 65536  # SPH_NOALLOC_MAX_PARTICLES
@@ -143,6 +144,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_hip_download_particles": (c_int, [c_void_p, c_void_p, c_void_p]),
         "sph_hip_set_graph": (c_int, [c_void_p, c_int]),
         "sph_hip_walk_dump": (c_int, [c_void_p] * 5),
+        "sph_hip_grid_info": (c_int, [c_void_p, POINTER(c_float), POINTER(c_double)]),
+        "sph_batch_download_particles": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
+        "sph_batch_grid_info": (c_int, [c_void_p, c_uint32, POINTER(c_float), POINTER(c_double)]),
         "sph_batch_walk_dump": (c_int, [c_void_p, c_uint32] + [c_void_p] * 4),
         "sph_hip_count": (c_uint32, [c_void_p]),
         "sph_hip_profile": (
@@ -290,15 +294,9 @@ class Context:
     """Resident single scene: SPHSystem::update without per-step PCIe."""
 
     def __init__(self, n: int, s: Settings, box_: Box, dt: float, cell_scale: float = 1.0,
-                 keep_force: bool = False, lane_walk: bool = False):
+                 keep_force: bool = False, lane_walk: bool = False, auto_grid: bool = False):
         self._lib = load()
-        d = SceneDesc()
-        d.n = n
-        d.dt = dt
-        d.settings = s
-        d.box = box_
-        d.cell_scale = cell_scale
-        d.flags = (SPH_SCENE_KEEP_FORCE if keep_force else 0) | (SPH_SCENE_LANE_WALK if lane_walk else 0)
+        d = scene_desc(n, s, box_, dt, cell_scale, keep_force, lane_walk, auto_grid)
         h = c_void_p()
         _check(self._lib.sph_hip_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -359,6 +357,13 @@ class Context:
         _check(fn(self._h, trial_steps, ctypes.byref(t)))
         return t.as_dict()
 
+    def grid_info(self):
+        """(current cell scale, candidates per neighbour of the last step's
+        state on the current grid)."""
+        sc, r = c_float(0), c_double(0)
+        _check(self._lib.sph_hip_grid_info(self._h, ctypes.byref(sc), ctypes.byref(r)))
+        return float(sc.value), float(r.value)
+
     def walk_dump(self, lists: bool = False) -> dict:
         """Neighbour counts and walk classes of the production walks on the
         current state (sph_hip_walk_dump), by particle id."""
@@ -387,14 +392,15 @@ def _walk_dump(fn, n: int, lists: bool):
 
 
 def scene_desc(n: int, s: Settings, box_: Box, dt: float, cell_scale: float = 1.0,
-               keep_force: bool = False, lane_walk: bool = False) -> SceneDesc:
+               keep_force: bool = False, lane_walk: bool = False, auto_grid: bool = False) -> SceneDesc:
     d = SceneDesc()
     d.n = n
     d.dt = dt
     d.settings = s
     d.box = box_
     d.cell_scale = cell_scale
-    d.flags = (SPH_SCENE_KEEP_FORCE if keep_force else 0) | (SPH_SCENE_LANE_WALK if lane_walk else 0)
+    d.flags = ((SPH_SCENE_KEEP_FORCE if keep_force else 0) | (SPH_SCENE_LANE_WALK if lane_walk else 0) |
+               (SPH_SCENE_AUTO_GRID if auto_grid else 0))
     return d
 
 
@@ -463,6 +469,21 @@ class Batch:
         fn = self._lib.sph_batch_autotune_grid if grid else self._lib.sph_batch_autotune
         _check(fn(self._h, trial_steps, ctypes.byref(t)))
         return t.as_dict()
+
+    def download_particles(self, scene: int, transforms: bool = True):
+        """Reference-layout particles of one scene (device order) and, if
+        asked, its transforms."""
+        n = self.counts[scene]
+        p = np.zeros(n, dtype=PARTICLE_DTYPE)
+        xf = np.zeros((n, 16), dtype=np.float32) if transforms else None
+        _check(self._lib.sph_batch_download_particles(self._h, scene, _ptr(p), _ptr(xf)))
+        return p, xf
+
+    def grid_info(self, scene: int):
+        """Context.grid_info for one scene."""
+        sc, r = c_float(0), c_double(0)
+        _check(self._lib.sph_batch_grid_info(self._h, scene, ctypes.byref(sc), ctypes.byref(r)))
+        return float(sc.value), float(r.value)
 
     def walk_dump(self, scene: int, lists: bool = False) -> dict:
         """As Context.walk_dump, for one scene of the batch."""

@@ -31,9 +31,10 @@ def make_pool(size: int, seed: int = 1024):
     return pool
 
 
-def scene_desc(sc):
+def scene_desc(sc, auto_grid=False):
     return sph_amd.scene_desc(sc.n, sph_amd.settings(*sc.settings_args()),
-                              sph_amd.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt)
+                              sph_amd.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt,
+                              auto_grid=auto_grid)
 
 
 def run_stream(rate: float, duration: float, pool, seed: int = 1024, memory_budget: int = 0,

@@ -78,14 +78,20 @@ def test_batch_order_of_scenes_does_not_matter(sph):
 
 
 def test_batch_teacher_forced_parity(sph, orc32):
+    """Three batched scenes after 10 steps: the next batched step against
+    the oracle fed the same state -- density, pressure and force."""
     scenes = sph_scenes.batched_workload(nscenes=3, seed=5)
-    with sph.Batch([_desc(sph, sc) for sc in scenes]) as b:
+    descs = [sph.scene_desc(sc.n, sph.settings(*sc.settings_args()),
+                            sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt, keep_force=True)
+             for sc in scenes]
+    with sph.Batch(descs) as b:
         for k, sc in enumerate(scenes):
             b.upload(k, sc.pos, sc.vel)
         b.step(10)
         before = [b.download(k) for k in range(3)]
         b.step(1)
         after = [b.download(k) for k in range(3)]
+        forces = [b.download_particles(k, transforms=False)[0]["force"] for k in range(3)]
     for k, sc in enumerate(scenes):
         st = {key: np.zeros_like(v) for key, v in before[k].items() if key != "ids"}
         for key in st:
@@ -95,8 +101,12 @@ def test_batch_teacher_forced_parity(sph, orc32):
             nx[key][after[k]["ids"].astype(np.int64)] = after[k][key]
         s_o = orc32.settings(*sc.settings_args())
         box_o = orc32.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset)
-        ref, _, _, _, _ = pu.oracle_step(orc32, st["pos"], st["vel"], s_o, sc.dt, box_o)
+        ref, _, rows, _, _ = pu.oracle_step(orc32, st["pos"], st["vel"], s_o, sc.dt, box_o)
         pu.assert_rho_p(nx["rho"], nx["pressure"], ref["density"], ref["pressure"], s_o, sc.name)
+        f = np.zeros((sc.n, 3), dtype=F32)
+        f[after[k]["ids"].astype(np.int64)] = forces[k]
+        scale = pu.force_scale(st["pos"], st["vel"], ref["density"], ref["pressure"], rows, s_o)
+        pu.assert_force(f, ref["force"][:, :3].astype(np.float64), scale, sc.name)
 
 
 def _dense_cluster(n, h, seed):

@@ -201,3 +201,63 @@ def test_autotune_grid_restores_state_and_keeps_the_chosen_grid(sph):
         with _ctx(sph, s, t["cell_scale"]) as ctx:
             ctx.step(10)
             assert _bytes(ctx.download()) == mixed[k], f"scene {k}"
+
+
+def test_auto_grid_follows_the_state_and_keeps_batch_identity(sph, orc32):
+    """SPH_SCENE_AUTO_GRID: a jittered lattice (uniform at the scale of h)
+    moves to cells of h / 2 after its first step while a clumped scene
+    stays on cells of h (premises, from the sampled candidates per
+    neighbour); over 70 steps (two re-grid points) each scene is
+    bit-identical alone and in a batch beside scenes without the flag; the
+    step after that matches the oracle fed the same state."""
+    h = 0.03  # a 52^3 jittered lattice of spacing h / 2 (the dam break's start), above the size gate
+    pos = sph_scenes.lattice((52, 52, 52), 0.5 * h, [-0.5, -0.9, -0.5], 0.01 * h, 3)
+    lattice = sph_scenes.Scene("lattice-52", pos, np.zeros_like(pos), 1.6e-4, 1000.0,
+                               1.0, 1.04, h, -9.8, 0.2, 0.001, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0))
+    assert lattice.n >= 1 << 17
+    clumps = sph_scenes.blobs(nblobs=4, per_blob=2500)
+    other = sph_scenes.batched_workload(nscenes=2, seed=9)
+    scenes = [lattice, clumps] + other
+    auto = [True, True, False, False]
+
+    def desc(sc, a, keep_force=False):
+        return sph.scene_desc(sc.n, sph.settings(*sc.settings_args()),
+                              sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt,
+                              auto_grid=a, keep_force=keep_force)
+
+    with sph.Batch([desc(sc, a) for sc, a in zip(scenes, auto)]) as b:
+        for k, sc in enumerate(scenes):
+            b.upload(k, sc.pos, sc.vel)
+        b.step(1)
+        assert b.grid_info(0)[0] == 0.5, b.grid_info(0)
+        assert b.grid_info(1)[0] == 1.0, b.grid_info(1)
+        b.step(69)
+        mixed = [_bytes(b.download(k)) for k in range(2)]
+    for k in range(2):
+        sc = scenes[k]
+        with sph.Context(sc.n, sph.settings(*sc.settings_args()),
+                         sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt, auto_grid=True) as ctx:
+            ctx.upload(sc.pos, sc.vel)
+            ctx.step(70)
+            assert _bytes(ctx.download()) == mixed[k], f"scene {k} ({sc.name}) differs alone"
+    # parity of the fine grid the lattice chose: one step from its state
+    sc = lattice
+    with sph.Context(sc.n, sph.settings(*sc.settings_args()),
+                     sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt, auto_grid=True,
+                     keep_force=True) as ctx:
+        ctx.upload(sc.pos, sc.vel)
+        ctx.step(70)
+        assert ctx.grid_info()[0] == 0.5
+        st = _by_id(ctx.download())
+        ctx.step(1)
+        p, _ = ctx.download_particles(transforms=False)
+        d = ctx.download()
+    nxt = _by_id(d)
+    f = np.zeros((sc.n, 3), dtype=F32)
+    f[d["ids"].astype(np.int64)] = p["force"][:, :3]
+    s_o = orc32.settings(*sc.settings_args())
+    box_o = orc32.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset)
+    ref, _, rows, _, _ = pu.oracle_step(orc32, st["pos"], st["vel"], s_o, sc.dt, box_o)
+    pu.assert_rho_p(nxt["rho"], nxt["pressure"], ref["density"], ref["pressure"], s_o, "auto grid")
+    scale = pu.force_scale(st["pos"], st["vel"], ref["density"], ref["pressure"], rows, s_o)
+    pu.assert_force(f, ref["force"][:, :3].astype(np.float64), scale, "auto grid")
