@@ -277,17 +277,27 @@ def batched_leg(nscenes=256, steps=500, tune_steps=5, seed=1024, auto_grid=True)
     }
 
 
-def serving_leg(rates, window, budget, pool_size=12):
-    pool = sph_serving.make_pool(pool_size)
+def serving_leg(rates, window, budget, pregen=240, max_queued=64, drain_s=60.0):
+    """The serving loop under Poisson streams at several rates: every request
+    on a scene of its own drawn size (the first `pregen` requests' scenes are
+    generated before the windows, shared by all rates); the server refuses
+    arrivals while max_queued requests wait; after each window the loop
+    drains, so latencies cover every accepted request."""
+    t0 = time.perf_counter()
+    scenes = sph_serving.pregenerate(pregen)
+    log(f"serving: {len(scenes)} request scenes generated in {time.perf_counter() - t0:.1f} s")
     out = []
     for r in rates:
-        res = sph_serving.run_stream(rate=r, duration=window, pool=pool, memory_budget=budget)
+        res = sph_serving.run_stream(rate=r, duration=window, scenes=scenes, memory_budget=budget,
+                                     max_queued=max_queued, drain_s=drain_s)
         log(f"serving at {r} req/s: " + ", ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}"
                                                    for k, v in res.items()))
         out.append(res)
     return {
-        "requests": "Poisson arrivals; particles log-uniform [2k, 500k] (pool of "
-                    f"{pool_size} pre-generated scenes), steps U[10, 1000], dam-break or droplet",
+        "requests": "Poisson arrivals; each request's particles log-uniform [2k, 500k] (its own scene at that "
+                    "size), steps U[10, 1000], dam-break or droplet; server queue limit "
+                    f"{max_queued} (arrivals beyond are refused and counted), backfill admission, drained "
+                    "after the window",
         "memory_budget_bytes": budget,
         "window_s_per_rate": window,
         "rates": out,

@@ -305,10 +305,17 @@ typedef struct sph_server_config_v1 {
     uint32_t max_scenes;         /* admitted scenes at once; 0 = 1024 */
     uint32_t max_steps_per_tick; /* batch steps per tick; 0 = 8, at most 256 */
     uint32_t flags;              /* SPH_SERVER_* bits, others must be 0 */
+    uint32_t max_queued;         /* queued requests at once; 0 = no limit; a submit
+                                    beyond it fails with SPH_E_STATE (counted as rejected) */
 } sph_server_config_v1;
 
 /* Step with stream launches instead of one captured graph per step. */
 #define SPH_SERVER_NO_GRAPH 1u
+/* Strict first-come first-served admission.  By default, when the head of
+ * the queue does not fit, a later request that fits is admitted ahead of it
+ * if it will finish (in batch steps) before enough memory frees up for the
+ * head, so backfilling never delays the head. */
+#define SPH_SERVER_NO_BACKFILL 2u
 
 #define SPH_REQ_QUEUED 0
 #define SPH_REQ_ACTIVE 1
@@ -324,6 +331,8 @@ typedef struct sph_server_stats_v1 {
     double busy_ms;              /* GPU time of all batch steps */
     double wall_s;               /* seconds since create */
     double step_ms_p50, step_ms_p99, step_ms_max;  /* per batch step */
+    uint64_t rejected;           /* submits refused: queue at max_queued */
+    uint64_t backfilled;         /* requests admitted ahead of the queue's head */
 } sph_server_stats_v1;
 
 int sph_server_create(const sph_server_config_v1 *cfg, sph_server **out);

@@ -43,7 +43,7 @@ struct Server {
     std::vector<float> step_ms;  // GPU time of every batch step
     double t0 = 0;
     double particle_steps = 0, scene_steps = 0, busy_ms = 0;
-    uint64_t completed = 0;
+    uint64_t completed = 0, rejected = 0, backfilled = 0;
 
     ~Server();
     int init(const sph_server_config_v1 *c);
@@ -52,6 +52,7 @@ struct Server {
     int submit(const sph_scene_desc_v1 &d, const float *pos, const float *vel, uint32_t steps, uint64_t *id);
     int tick(uint32_t *ran_steps);
     void requeue(const std::vector<Request *> &join);
+    void backfill(const std::vector<Request *> &keep, std::vector<Request *> &join);
     int fetch(uint64_t id, float *pos, float *vel, float *rho, float *pressure, uint32_t *ids);
     int stats(sph_server_stats_v1 *out) const;
 };

@@ -65,7 +65,7 @@ int Server::init(const sph_server_config_v1 *c)
     int rc;
     if (c)
         cfg = *c;
-    if (cfg.flags & ~SPH_SERVER_NO_GRAPH)
+    if (cfg.flags & ~(SPH_SERVER_NO_GRAPH | SPH_SERVER_NO_BACKFILL))
         return fail(SPH_E_INVALID, "server: unknown flags 0x%x", cfg.flags);
     if ((rc = batch.init_empty(false)))
         return rc;
@@ -97,6 +97,10 @@ int Server::submit(const sph_scene_desc_v1 &d, const float *pos, const float *ve
         return fail(SPH_E_INVALID, "submit: NULL positions or id");
     if (steps == 0)
         return fail(SPH_E_INVALID, "submit: steps must be > 0");
+    if (cfg.max_queued && queue.size() >= cfg.max_queued) {
+        ++rejected;
+        return fail(SPH_E_STATE, "submit: %zu requests queued (the limit)", queue.size());
+    }
     if ((d.flags & SPH_SCENE_KEEP_FORCE) != 0u)
         return fail(SPH_E_INVALID, "submit: the serving loop keeps no forces");
     std::unique_ptr<Request> r(new (std::nothrow) Request());
@@ -129,15 +133,56 @@ int Server::submit(const sph_scene_desc_v1 &d, const float *pos, const float *ve
     return SPH_OK;
 }
 
-// Puts the requests of a failed admission back at the head of the queue,
-// in their order, and releases their charge.
+// The head of the queue does not fit.  Memory frees up as running requests
+// finish: `shadow` is the number of batch steps until enough has freed for
+// the head.  A later request is admitted now when it fits and will finish
+// within those steps, so it never delays the head.  Backfilled requests
+// keep their queue order among themselves.
+void Server::backfill(const std::vector<Request *> &keep, std::vector<Request *> &join)
+{
+    const Request *head = queue.front();
+    if (head->bytes > cfg.memory_budget)
+        return;
+    std::vector<std::pair<uint64_t, uint64_t>> ends;  // (steps left, bytes) of what runs
+    for (const Request *r : keep)
+        ends.emplace_back(r->steps - r->done, r->bytes);
+    for (const Request *r : join)
+        ends.emplace_back(r->steps, r->bytes);
+    std::sort(ends.begin(), ends.end());
+    uint64_t free_b = cfg.memory_budget - used_bytes, shadow = UINT64_MAX;
+    for (const auto &e : ends) {
+        free_b += e.second;
+        if (free_b >= head->bytes) {
+            shadow = e.first;
+            break;
+        }
+    }
+    if (shadow == UINT64_MAX || shadow == 0)
+        return;
+    for (auto it = queue.begin() + 1; it != queue.end() && keep.size() + join.size() < cfg.max_scenes;) {
+        Request *r = *it;
+        if (used_bytes + r->bytes <= cfg.memory_budget && r->steps <= shadow) {
+            used_bytes += r->bytes;
+            join.push_back(r);
+            ++backfilled;
+            it = queue.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+// Puts the requests of a failed admission back into the queue, in submit
+// order (backfilled ones behind the head they passed), and releases their
+// charge.
 void Server::requeue(const std::vector<Request *> &join)
 {
-    for (auto it = join.rbegin(); it != join.rend(); ++it) {
-        (*it)->state = SPH_REQ_QUEUED;
-        used_bytes -= (*it)->bytes;
-        queue.push_front(*it);
+    for (Request *r : join) {
+        r->state = SPH_REQ_QUEUED;
+        used_bytes -= r->bytes;
+        queue.push_back(r);
     }
+    std::sort(queue.begin(), queue.end(), [](const Request *a, const Request *b) { return a->id < b->id; });
 }
 
 int Server::tick(uint32_t *ran_steps)
@@ -160,6 +205,8 @@ int Server::tick(uint32_t *ran_steps)
         used_bytes += r->bytes;
         join.push_back(r);
     }
+    if (!queue.empty() && !(cfg.flags & SPH_SERVER_NO_BACKFILL) && keep.size() + join.size() < cfg.max_scenes)
+        backfill(keep, join);
     if (keep.size() != active.size() || !join.empty()) {
         std::vector<BatchScene> next;
         std::vector<int> from;
@@ -297,6 +344,8 @@ int Server::stats(sph_server_stats_v1 *o) const
     o->step_ms_p50 = percentile(step_ms, 0.50);
     o->step_ms_p99 = percentile(step_ms, 0.99);
     o->step_ms_max = step_ms.empty() ? 0.0 : *std::max_element(step_ms.begin(), step_ms.end());
+    o->rejected = rejected;
+    o->backfilled = backfilled;
     return SPH_OK;
 }
 

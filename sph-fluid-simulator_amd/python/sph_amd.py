@@ -30,6 +30,7 @@ SPH_E_HIP = -4
 SPH_E_LIMIT = -5
 SPH_E_STATE = -6
 SPH_SCENE_AUTO_GRID = 4
+SPH_SERVER_NO_BACKFILL = 2
 SPH_SERVER_FAIL_RELAYOUT = 1
 SPH_SERVER_FAIL_UPLOAD = 2
 NOALLOC_MAX_PARTICLES = 65536  # SPH_NOALLOC_MAX_PARTICLES
@@ -89,14 +90,14 @@ class Tune(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("memory_budget", c_uint64), ("max_scenes", c_uint32), ("max_steps_per_tick", c_uint32),
-                ("flags", c_uint32)]
+                ("flags", c_uint32), ("max_queued", c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, c_uint64) for k in ("submitted", "admitted", "completed", "queued", "active", "steps",
                                         "used_bytes", "budget_bytes")] + [
         (k, c_double) for k in ("particle_steps", "scene_steps", "busy_ms", "wall_s", "step_ms_p50",
-                                "step_ms_p99", "step_ms_max")]
+                                "step_ms_p99", "step_ms_max")] + [(k, c_uint64) for k in ("rejected", "backfilled")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -503,9 +504,10 @@ class Server:
     requests, call tick() to run scheduling rounds, fetch finished results."""
 
     def __init__(self, memory_budget: int = 0, max_scenes: int = 0, max_steps_per_tick: int = 0,
-                 graph: bool = True):
+                 graph: bool = True, max_queued: int = 0, backfill: bool = True):
         self._lib = load()
-        cfg = ServerConfig(memory_budget, max_scenes, max_steps_per_tick, 0 if graph else SPH_SERVER_NO_GRAPH)
+        flags = (0 if graph else SPH_SERVER_NO_GRAPH) | (0 if backfill else SPH_SERVER_NO_BACKFILL)
+        cfg = ServerConfig(memory_budget, max_scenes, max_steps_per_tick, flags, max_queued)
         h = c_void_p()
         _check(self._lib.sph_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

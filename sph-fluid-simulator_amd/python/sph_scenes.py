@@ -206,14 +206,17 @@ def request_scene(n: int, kind: str, seed: int, dt: float = 0.001) -> Scene:
 
 
 def request_stream(rate: float, duration: float, seed: int = 1024, n_lo: int = 2000, n_hi: int = 500000,
-                   steps_lo: int = 10, steps_hi: int = 1000):
-    """Poisson arrivals over [0, duration) at `rate` per second: a list of
-    (arrival seconds, particle count log-uniform in [n_lo, n_hi], steps
-    uniform in [steps_lo, steps_hi], geometry dambreak or droplet, seed)."""
+                   steps_lo: int = 10, steps_hi: int = 1000, max_requests: int | None = None):
+    """Poisson arrivals over [0, duration) at `rate` per second (at most
+    max_requests of them): a list of (arrival seconds, particle count
+    log-uniform in [n_lo, n_hi], steps uniform in [steps_lo, steps_hi],
+    geometry dambreak or droplet, seed).  Every field but the arrival time
+    is drawn in the same order at any rate, so request i is the same
+    request at every rate."""
     rng = np.random.default_rng(seed)
     out = []
     t = 0.0
-    while True:
+    while max_requests is None or len(out) < max_requests:
         t += float(rng.exponential(1.0 / rate))
         if t >= duration:
             break

@@ -225,3 +225,9 @@ def test_invalid_cell_scale_rejected(sph, scale):
     h = ctypes.c_void_p()
     assert sph.load().sph_hip_create(ctypes.byref(d), ctypes.byref(h)) == sph.SPH_E_INVALID
     assert b"cell_scale" in sph.load().sph_hip_last_error()
+
+
+def test_server_structs_match_the_header(sph):
+    # uint64 + 4 x uint32; 8 x uint64 + 7 x double + 2 x uint64
+    assert ctypes.sizeof(sph.ServerConfig) == 24
+    assert ctypes.sizeof(sph.ServerStats) == 136
