@@ -3,8 +3,10 @@
 // of lijenicol/SPH-Fluid-Simulator; stepping runs on the device.
 #include "SPHSystem.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 namespace {
@@ -87,6 +89,8 @@ void SPHSystem::initParticles()
             }
         }
     }
+    particleIds.resize(particleCount);
+    std::iota(particleIds.begin(), particleIds.end(), 0u);
 }
 
 void SPHSystem::createContext()
@@ -106,12 +110,19 @@ void SPHSystem::createContext()
     d.cell_scale = 1.0f;
     d.flags = SPH_SCENE_KEEP_FORCE;
     check(sph_hip_create(&d, &ctx));
+    // the device numbers particles 0..n-1 in upload order: that number is
+    // the id; the caller's accelerations are kept per id
     std::vector<float> pos(3 * particleCount), vel(3 * particleCount);
+    std::vector<float> acc(3 * particleCount);
     for (size_t i = 0; i < particleCount; ++i)
         for (int a = 0; a < 3; ++a) {
             pos[3 * i + a] = particles[i].position[a];
             vel[3 * i + a] = particles[i].velocity[a];
+            acc[3 * i + a] = particles[i].acceleration[a];
         }
+    accelerationById.swap(acc);
+    particleIds.resize(particleCount);
+    std::iota(particleIds.begin(), particleIds.end(), 0u);
     check(sph_hip_upload(ctx, pos.data(), vel.data()));
 }
 
@@ -125,10 +136,31 @@ void SPHSystem::update(float deltaTime)
         syncHost();
 }
 
+// Device order -> the reference's order: sorted by 16-bit bucket, ties by
+// id; acceleration restored from the caller's values.
 void SPHSystem::syncHost()
 {
-    if (ctx)
-        check(sph_hip_download_particles(ctx, particles, sphereModelMtxs));
+    if (!ctx)
+        return;
+    const size_t n = particleCount;
+    std::vector<Particle> dev(n);
+    std::vector<float> xf(16 * n);
+    std::vector<uint32_t> ids(n);
+    check(sph_hip_download_particles(ctx, dev.data(), xf.data()));
+    check(sph_hip_download(ctx, nullptr, nullptr, nullptr, nullptr, ids.data()));
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return dev[a].hash != dev[b].hash ? dev[a].hash < dev[b].hash : ids[a] < ids[b];
+    });
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t s = order[k], id = ids[s];
+        particles[k] = dev[s];
+        for (int a = 0; a < 3; ++a)
+            particles[k].acceleration[a] = accelerationById[3 * (size_t)id + a];
+        std::memcpy(sphereModelMtxs + 16 * k, xf.data() + 16 * (size_t)s, 16 * sizeof(float));
+        particleIds[k] = id;
+    }
 }
 
 void SPHSystem::reset()

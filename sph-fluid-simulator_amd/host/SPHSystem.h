@@ -12,13 +12,19 @@
 // into the instanced VBO (src/SPHSystem.cpp:119-133).
 //
 // After update() the device state is authoritative and the host copies
-// (particles, transforms) are refreshed from it, in device order: like the
-// reference, update() permutes the particle array.  With lazy host sync
-// enabled, update() leaves the host copies stale until syncHost().
+// (particles, transforms) are refreshed from it.  As in the reference
+// (src/sph.cpp:184-192), update() leaves the particle array sorted by the
+// 16-bit bucket `hash` (ties in the order of ids(), one valid outcome of the
+// reference's unstable std::sort) with transforms[i] belonging to
+// particles[i]; each particle's `acceleration` comes back as the caller left
+// it (the step never reads or writes it).  With lazy host sync enabled,
+// update() leaves the host copies stale until syncHost().
 #pragma once
 
 #include <cstddef>
+#include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/sph_hip.h"
 
@@ -51,6 +57,9 @@ public:
 
     // Instance transforms of the particles (16 floats each, column-major).
     const float *transforms() const { return sphereModelMtxs; }
+    // ids()[i]: the index particles[i] had when the system was built (or
+    // reset) -- an extension, the reference's Particle carries no id.
+    const uint32_t *ids() const { return particleIds.data(); }
 
     // Extensions: lazy host copies, explicit sync, an arbitrary wall box.
     void setLazyHostSync(bool lazy) { lazyHostSync = lazy; }
@@ -67,6 +76,8 @@ private:
     sph_box_v1 box;
     float *sphereModelMtxs;
     sph_ctx *ctx = nullptr;
+    std::vector<uint32_t> particleIds;     // id of particles[i]
+    std::vector<float> accelerationById;   // acceleration the caller left, per id (3 floats)
 
     void initParticles();
     void createContext();

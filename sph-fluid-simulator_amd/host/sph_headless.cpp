@@ -2,12 +2,15 @@
 // reference application's simulation (src/Tester.cpp:90-91,110-125) without
 // window, GUI or renderer.
 //
-//   sph_headless [--width W] [--steps S] [--gpu-box] [--lazy] [--dump FILE]
+//   sph_headless [--width W] [--steps S] [--gpu-box] [--lazy]
+//                [--reset-after R] [--dump FILE]
 //
 // Builds SPHSystem(W, SPHSettings(0.02, 1000, 1, 1.04, 0.15, -9.8, 0.2),
 // runOnGPU), starts it, calls update() S times and prints the mean step
-// time.  --dump writes the particle array and the transform buffer:
-// uint64 n, n * 60-byte Particle records, n * 16 floats.
+// time.  --reset-after R first runs R updates, calls reset() and update()
+// once more (the start gate is closed again: a no-op) before starting
+// over.  --dump writes the particle array, the transform buffer and the
+// ids: uint64 n, n * 60-byte Particle records, n * 16 floats, n * uint32.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -21,6 +24,7 @@ int main(int argc, char **argv)
     size_t width = 15;
     int steps = 100;
     bool gpuBox = false, lazy = false;
+    int resetAfter = -1;
     std::string dump;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -34,8 +38,11 @@ int main(int argc, char **argv)
             lazy = true;
         else if (a == "--dump" && i + 1 < argc)
             dump = argv[++i];
+        else if (a == "--reset-after" && i + 1 < argc)
+            resetAfter = std::atoi(argv[++i]);
         else {
-            std::fprintf(stderr, "usage: %s [--width W] [--steps S] [--gpu-box] [--lazy] [--dump FILE]\n", argv[0]);
+            std::fprintf(stderr, "usage: %s [--width W] [--steps S] [--gpu-box] [--lazy] [--reset-after R] "
+                                 "[--dump FILE]\n", argv[0]);
             return 2;
         }
     }
@@ -44,6 +51,13 @@ int main(int argc, char **argv)
         SPHSystem sys(width, settings, gpuBox);
         sys.setLazyHostSync(lazy);
         sys.update(0.016f);  // not started: a no-op, as in the reference
+        if (resetAfter >= 0) {
+            sys.startSimulation();
+            for (int s = 0; s < resetAfter; ++s)
+                sys.update(0.016f);
+            sys.reset();
+            sys.update(0.016f);  // the gate is closed again: a no-op
+        }
         sys.startSimulation();
         auto t0 = std::chrono::steady_clock::now();
         for (int s = 0; s < steps; ++s)
@@ -64,6 +78,7 @@ int main(int argc, char **argv)
             std::fwrite(&n, sizeof n, 1, f);
             std::fwrite(sys.particles, sizeof(Particle), n, f);
             std::fwrite(sys.transforms(), 16 * sizeof(float), n, f);
+            std::fwrite(sys.ids(), sizeof(uint32_t), n, f);
             std::fclose(f);
         }
     } catch (const SPHError &e) {

@@ -107,8 +107,9 @@ def collision_clump(n=600, h=0.05, seed=9):
     reference cells (0,0,0), (0,1,-1), (0,1,0) and (0,0,-1).  The buckets of
     (0,-1,1) and (0,1,-1) coincide, so particles of cell (0,0,0) are slow
     and count their neighbours in (0,1,-1) twice.  Every lane's stencil
-    holds the whole clump, so waves take the union walk, and n is small
-    enough that no wave task is split."""
+    holds the whole clump: waves whose lanes are all fast take the union
+    walk, waves holding a slow lane the per-lane walk (which applies m_ij
+    per neighbour), and n is small enough that no wave task is split."""
     rng = np.random.default_rng(seed)
     centre = np.array([0.3 * h, h, -h])
     pos = (centre + rng.uniform(-0.2 * h, 0.2 * h, size=(n, 3))).astype(F32)

@@ -87,11 +87,43 @@ def test_fine_grid_force_and_slow_lanes(sph, orc32, cell_scale):
     _one_step_parity(sph, orc32, sc, cell_scale, f"clump scale {cell_scale:.3f}")
 
 
+def _dense_slab(cell_scale, h=0.05):
+    """A slab 32 x 2 x 1 grid cells (cell edge h (1 + 2^-10) cell_scale)
+    holding 2 x 2 x 2 particles per cell: in cell order a wave's 64 slots
+    span 8 cells in x, so its union of stencils is twice its own and it
+    takes the union walk; its few stencil rows keep every walk below the
+    part size."""
+    H = h * (1.0 + 2.0 ** -10) * cell_scale
+    a = 0.5 * H
+    pos = sph_scenes.lattice((64, 4, 2), a, [-0.5 + 0.25 * a, 0.1 + 0.25 * a, 0.2 + 0.25 * a], 0.01 * a, 17)
+    mass = float(F32(0.02 * (h / 0.15) ** 3))
+    # a tiny time step: the slab is far denser than its rest density and
+    # would fly apart, changing which tasks split, within a few steps
+    # a box tight around it: a scene's grid holds at most 64 cells per
+    # particle, and a wider box would coarsen these cells
+    return sph_scenes.Scene("dense-slab", pos, np.zeros_like(pos), mass, 1000.0, 1.0, 1.04, h, -9.8, 0.2,
+                            1e-6, (-0.6, 0.0, 0.1), (0.2, 0.25, 0.35), steps=10)
+
+
 @pytest.mark.parametrize("cell_scale", SCALES)
 def test_fine_grid_union_matches_lane_walk(sph, cell_scale):
-    """Unsplit wave tasks: the union and per-lane walks give bit-identical
-    states step after step on a fine grid, with and without slow lanes."""
-    for sc, every in ((sph_scenes.workload(0), 15), (pu.collision_clump(), 2)):
+    """The union and per-lane walks give bit-identical states step after
+    step on a fine grid: on the 8k cube and on a dense slab whose waves span
+    8 cells in x (premise: union walks occur there and no task is split),
+    and on the slow-lane clump (its waves all hold a slow lane, so both runs
+    walk per lane; on fine grids its tasks are split into parts)."""
+    scenes = ((sph_scenes.workload(0), 15), (pu.collision_clump(), 2), (_dense_slab(cell_scale), 2))
+    unions = 0
+    for sc, every in (scenes[0], scenes[2]):
+        for lane_walk in (False, True):
+            with _ctx(sph, sc, cell_scale, lane_walk=lane_walk) as ctx:
+                for _ in range(3):  # every compared state, both runs
+                    dump = ctx.walk_dump()
+                    unions += 0 if lane_walk else int(dump["union"].sum())
+                    assert not dump["split"].any(), f"{sc.name}: premise, no task is split"
+                    ctx.step(every)
+    assert unions > 0, "premise: some waves take the union walk"
+    for sc, every in scenes:
         states = []
         for lane_walk in (False, True):
             with _ctx(sph, sc, cell_scale, lane_walk=lane_walk) as ctx:

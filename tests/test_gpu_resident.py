@@ -195,8 +195,17 @@ def _blob_states(sph, sc, lane_walk, chunks=4, every=10):
 def test_union_walk_matches_lane_walk_unsplit(sph):
     """Where no wave task is split (every walk below the part size), the
     wave-uniform candidate walk and the per-lane walk give bit-identical
-    states step after step."""
+    states step after step (premise: union walks occur along the run and no
+    task is split)."""
     sc = sph_scenes.workload(0)
+    with make_ctx(sph, sc) as ctx:
+        unions = 0
+        for _ in range(3):
+            ctx.step(15)
+            dump = ctx.walk_dump()
+            unions += int(dump["union"].sum())
+            assert not dump["split"].any(), "premise: no task is split"
+    assert unions > 0, "premise: some waves take the union walk"
     u = _blob_states(sph, sc, False, chunks=3, every=15)
     l = _blob_states(sph, sc, True, chunks=3, every=15)
     for k, (x, y) in enumerate(zip(u, l)):
@@ -237,10 +246,13 @@ def test_split_walks_deterministic_and_agree(sph, orc32):
     assert np.all(dv <= 1e-3 * dv_ref + 1e-5), float(np.max(dv - 1e-3 * dv_ref))
 
 
-def test_union_walk_with_slow_lanes(sph, orc32):
-    """Waves holding slow lanes (colliding buckets: m_ij = 2) take the union
-    walk: one step matches the oracle (density, pressure, force), and the
-    union and per-lane walks stay bit-identical over several steps."""
+def test_slow_lane_clump_parity_and_walk_identity(sph, orc32):
+    """A clump with slow lanes (colliding buckets: m_ij = 2): waves holding
+    a slow lane take the per-lane walk, which applies the exact
+    multiplicities (premise: every wave of this clump holds one, so all
+    walk per lane; their long walks are split into parts).  One step
+    matches the oracle (density, pressure, force), and the run stays
+    bit-identical to one with the per-lane walk forced."""
     import brute
     sc = pu.collision_clump()
     cells = brute.ref_cells(sc.pos, sc.h)
@@ -253,6 +265,8 @@ def test_union_walk_with_slow_lanes(sph, orc32):
     box_o = orc32.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset)
     ref, _, orows, _, _ = pu.oracle_step(orc32, sc.pos, sc.vel, s_o, sc.dt, box_o)
     with make_ctx(sph, sc, keep_force=True) as ctx:
+        dump = ctx.walk_dump()
+        assert not dump["union"].any(), "premise: per-lane walks"
         ctx.step(1)
         p, _ = ctx.download_particles(transforms=False)
         d = ctx.download()

@@ -231,3 +231,38 @@ def test_server_structs_match_the_header(sph):
     # uint64 + 4 x uint32; 8 x uint64 + 7 x double + 2 x uint64
     assert ctypes.sizeof(sph.ServerConfig) == 24
     assert ctypes.sizeof(sph.ServerStats) == 136
+
+
+def _glibc_rand(seed, count):
+    """glibc rand() (random() of TYPE_3, the default state) restated from its
+    published algorithm: r0 = seed; r_i = 16807 r_(i-1) mod (2^31 - 1) for
+    i = 1..30; r_i = r_(i-31) for i = 31..33; r_i = r_(i-31) + r_(i-3)
+    mod 2^32 after that; the k-th output is r_(k+344) >> 1."""
+    r = [seed]
+    for i in range(1, 31):
+        r.append((16807 * r[i - 1]) % 2147483647)
+    for i in range(31, 34):
+        r.append(r[i - 31])
+    for i in range(34, 344 + count):
+        r.append((r[i - 31] + r[i - 3]) & 0xFFFFFFFF)
+    return [v >> 1 for v in r[344:344 + count]]
+
+
+def test_reference_initial_state_pinned_by_a_restated_glibc_rand():
+    """sph_scenes.reference_init (SPHSystem::initParticles, src/SPHSystem.cpp:
+    76-108, on libc's rand) equals the same state built on the restated
+    glibc generator, whose first outputs for seed 1 are the well-known
+    1804289383, 846930886, 1681692777."""
+    import sph_scenes
+    assert _glibc_rand(1, 3) == [1804289383, 846930886, 1681692777]
+    w, h = 9, F32(0.15)
+    seq = iter(_glibc_rand(1024, 3 * w ** 3))
+    pos = np.zeros((w ** 3, 3), dtype=F32)
+    sep = h + F32(0.01)
+    for i in range(w):
+        for j in range(w):
+            for k in range(w):
+                r = [(F32(next(seq)) / F32(2147483647) * F32(0.5) - F32(1)) * h / F32(10) for _ in range(3)]
+                pos[i + (j + w * k) * w] = (F32(i) * sep + r[0] - F32(1.5), F32(j) * sep + r[1] + h + F32(0.1),
+                                            F32(k) * sep + r[2] - F32(1.5))
+    assert np.array_equal(pos, sph_scenes.reference_init(w, 0.15))
