@@ -214,6 +214,15 @@ int sph_hip_walk_dump(sph_ctx *ctx, uint32_t *count, uint32_t *walk,
  * state, computed now on the current grid (0 before the first step): the
  * statistic SPH_SCENE_AUTO_GRID decides by.  Either pointer may be NULL. */
 int sph_hip_grid_info(sph_ctx *ctx, float *cell_scale, double *cand_per_nbr);
+/* The determinism limit: a cell holding more than 16,384 particles is not
+ * sorted by particle id and keeps the arrival order of its particles, so
+ * the summation order over it, and the last bits of the results near it,
+ * can differ between runs.  *out = the number of such cells summed over
+ * every cell sort the context ran since creation (steps, autotune trials,
+ * walk dumps); 0 means no step so far met the limit.  No counterpart in
+ * the reference (its std::sort of the particles by hash,
+ * src/sph.cpp:184-191, has no such limit). */
+int sph_hip_unordered_cells(sph_ctx *ctx, uint64_t *out);
 /* 1 = replay steps through a captured HIP graph (default), 0 = launch. */
 int sph_hip_set_graph(sph_ctx *ctx, int enable);
 uint32_t sph_hip_count(const sph_ctx *ctx);
@@ -244,6 +253,8 @@ int sph_batch_download_particles(sph_batch *b, uint32_t scene,
 /* sph_hip_grid_info for one scene of a batch. */
 int sph_batch_grid_info(sph_batch *b, uint32_t scene, float *cell_scale,
                         double *cand_per_nbr);
+/* sph_hip_unordered_cells for a batch (all its scenes). */
+int sph_batch_unordered_cells(sph_batch *b, uint64_t *out);
 /* sph_hip_walk_dump for one scene of a batch. */
 int sph_batch_walk_dump(sph_batch *b, uint32_t scene, uint32_t *count,
                         uint32_t *walk, uint32_t *list_count, uint32_t *list);

@@ -99,6 +99,9 @@ struct Batch {
     // re-grids those whose choice changed.
     int auto_grid();
     int grid_info(uint32_t s, float *cell_scale, double *cand_per_nbr);
+    // Cells the in-cell sort left in arrival order (more than
+    // kBigCellSortMax particles), summed over every sort since creation.
+    int unordered_cells(uint64_t *out);
     // step() with an event after every step; ms[k] (may be NULL) receives
     // the GPU time of step k.  Synchronises.
     int step_timed(int nsteps, float *ms);

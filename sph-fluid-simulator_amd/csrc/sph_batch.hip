@@ -444,6 +444,23 @@ int Batch::grid_info(uint32_t s, float *cell_scale, double *cand_per_nbr)
     return SPH_OK;
 }
 
+int Batch::unordered_cells(uint64_t *out)
+{
+    if (!out)
+        return fail(SPH_E_INVALID, "unordered_cells: NULL output");
+    *out = 0;
+    if (!w.unordered.ptr)
+        return SPH_OK;
+    unsigned long long v = 0;
+    hipError_t e = hipMemcpyAsync(&v, w.unordered.ptr, sizeof(v), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "reading the unordered-cell counter");
+    *out = (uint64_t)v;
+    return SPH_OK;
+}
+
 int Batch::auto_grid()
 {
     int rc;
@@ -987,6 +1004,20 @@ int sph_batch_grid_info(sph_batch *b, uint32_t scene, float *cell_scale, double 
     if (!b)
         return fail(SPH_E_INVALID, "grid_info: NULL batch");
     return b->b.grid_info(scene, cell_scale, cand_per_nbr);
+}
+
+int sph_hip_unordered_cells(sph_ctx *c, uint64_t *out)
+{
+    if (!c)
+        return fail(SPH_E_INVALID, "unordered_cells: NULL context");
+    return c->b.unordered_cells(out);
+}
+
+int sph_batch_unordered_cells(sph_batch *b, uint64_t *out)
+{
+    if (!b)
+        return fail(SPH_E_INVALID, "unordered_cells: NULL batch");
+    return b->b.unordered_cells(out);
 }
 
 int sph_batch_walk_dump(sph_batch *b, uint32_t scene, uint32_t *count, uint32_t *walk, uint32_t *list_count,

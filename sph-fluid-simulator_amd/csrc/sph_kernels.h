@@ -63,6 +63,7 @@ struct CellArrays {
     uint32_t *partials;
     uint32_t *big;     // [0] = number of big cells, then their indices (may be null)
     uint32_t big_cap;  // capacity of the index list
+    unsigned long long *unordered = nullptr;  // += 1 per cell and step kept in arrival order (may be null)
     uint32_t ncells_padded;  // multiple of kScanChunk
 };
 

@@ -504,7 +504,7 @@ __device__ inline void sort_cell(uint32_t *__restrict__ perm, uint32_t b, uint32
 
 __global__ __launch_bounds__(kBigSortBlock) void k_sort_big_cells(
     uint32_t *__restrict__ perm, const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ big,
-    uint32_t big_cap)
+    uint32_t big_cap, unsigned long long *__restrict__ unordered)
 {
     static_assert(kBigSortRegs == 16 && kSmallCell >= 64, "sort layout");
     __shared__ uint32_t sh[kBigCellSortMax];
@@ -512,8 +512,11 @@ __global__ __launch_bounds__(kBigSortBlock) void k_sort_big_cells(
     for (uint32_t e = blockIdx.x; e < nbig; e += gridDim.x) {
         const uint32_t cell = big[1 + e];
         const uint32_t b = cstart[cell], len = cstart[cell + 1] - b;
-        if (len > kBigCellSortMax)
-            continue;  // uniform: keeps arrival order
+        if (len > kBigCellSortMax) {  // uniform: keeps arrival order, counted
+            if (threadIdx.x == 0 && unordered)
+                atomicAdd(unordered, 1ull);
+            continue;
+        }
         uint32_t m = 1;
         while (m < len)
             m <<= 1;
@@ -532,7 +535,7 @@ void launch_sort_big_cells(hipStream_t st, const ParticleArrays &a, const CellAr
     if (!c.big)
         return;
     hipLaunchKernelGGL(k_sort_big_cells, dim3(kBigSortBlocks), dim3(kBigSortBlock), 0, st, a.perm,
-                       c.cstart, c.big, c.big_cap);
+                       c.cstart, c.big, c.big_cap, c.unordered);
 }
 
 // Slot s holds A index i; its stable destination is the cell start plus the

@@ -54,6 +54,9 @@ struct Workspace {
     DevBuf pnparts, poff, ppartials, pmeasure, ptotal, ptasks, pqueue, pbuf, pdone, pwalk, pown;
     // neighbour lists the density pass writes and the force pass reads
     DevBuf nlist, ncount;
+    // cells left in arrival order (above the in-cell sort's limit), summed
+    // over steps
+    DevBuf unordered;
     uint32_t parts_padded = 0;
     uint32_t ncells_padded = 0;
     uint32_t big_cap = 0;

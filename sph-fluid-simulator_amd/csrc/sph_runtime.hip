@@ -247,6 +247,12 @@ int Workspace::reserve_particles(uint32_t n)
     const uint32_t cap = n / (kSmallCell + 1u) + 1u;
     if ((rc = big.reserve(4ull * (cap + 1u))))
         return rc;
+    if (!unordered.ptr) {
+        if ((rc = unordered.reserve(8)))
+            return rc;
+        if (hipMemset(unordered.ptr, 0, 8) != hipSuccess)
+            return fail(SPH_E_HIP, "clearing the unordered-cell counter");
+    }
     big_cap = (uint32_t)(big.bytes / 4 - 1);
     if ((rc = posA.reserve(sizeof(float4) * (size_t)n)) || (rc = velA.reserve(sizeof(float4) * (size_t)n)) ||
         (rc = posB.reserve(sizeof(float4) * (size_t)n)) || (rc = velB.reserve(sizeof(float4) * (size_t)n)) ||
@@ -354,6 +360,7 @@ CellArrays Workspace::cells() const
     c.big = (uint32_t *)big.ptr;
     c.big_cap = big_cap;
     c.ncells_padded = ncells_padded;
+    c.unordered = (unsigned long long *)unordered.ptr;
     return c;
 }
 
@@ -404,7 +411,7 @@ void Workspace::release_all()
     for (DevBuf *b : {&posA, &velA, &posB, &velB, &cellcA, &rankA, &cellcB, &perm, &rhoB, &posR, &velR,
                       &counts, &cstart, &partials, &big, &force, &params, &tasks, &pnparts, &poff,
                       &ppartials, &pmeasure, &ptotal, &ptasks, &pqueue, &pbuf, &pdone, &pwalk, &pown,
-                      &nlist, &ncount})
+                      &nlist, &ncount, &unordered})
         b->release();
     ncells_padded = 0;
     big_cap = 0;

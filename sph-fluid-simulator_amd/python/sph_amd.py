@@ -148,6 +148,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "sph_hip_grid_info": (c_int, [c_void_p, POINTER(c_float), POINTER(c_double)]),
         "sph_batch_download_particles": (c_int, [c_void_p, c_uint32, c_void_p, c_void_p]),
         "sph_batch_grid_info": (c_int, [c_void_p, c_uint32, POINTER(c_float), POINTER(c_double)]),
+        "sph_hip_unordered_cells": (c_int, [c_void_p, POINTER(c_uint64)]),
+        "sph_batch_unordered_cells": (c_int, [c_void_p, POINTER(c_uint64)]),
         "sph_batch_walk_dump": (c_int, [c_void_p, c_uint32] + [c_void_p] * 4),
         "sph_hip_count": (c_uint32, [c_void_p]),
         "sph_hip_profile": (
@@ -365,6 +367,13 @@ class Context:
         _check(self._lib.sph_hip_grid_info(self._h, ctypes.byref(sc), ctypes.byref(r)))
         return float(sc.value), float(r.value)
 
+    def unordered_cells(self) -> int:
+        """Cells of more than 16,384 particles that kept arrival order (not
+        bit-reproducible), summed over every cell sort so far."""
+        v = c_uint64(0)
+        _check(self._lib.sph_hip_unordered_cells(self._h, ctypes.byref(v)))
+        return int(v.value)
+
     def walk_dump(self, lists: bool = False) -> dict:
         """Neighbour counts and walk classes of the production walks on the
         current state (sph_hip_walk_dump), by particle id."""
@@ -485,6 +494,12 @@ class Batch:
         sc, r = c_float(0), c_double(0)
         _check(self._lib.sph_batch_grid_info(self._h, scene, ctypes.byref(sc), ctypes.byref(r)))
         return float(sc.value), float(r.value)
+
+    def unordered_cells(self) -> int:
+        """Context.unordered_cells over all scenes of the batch."""
+        v = c_uint64(0)
+        _check(self._lib.sph_batch_unordered_cells(self._h, ctypes.byref(v)))
+        return int(v.value)
 
     def walk_dump(self, scene: int, lists: bool = False) -> dict:
         """As Context.walk_dump, for one scene of the batch."""
