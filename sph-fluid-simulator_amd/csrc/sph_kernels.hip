@@ -649,6 +649,93 @@ __device__ inline void for_each_row(const uint32_t *__restrict__ cstart, uint32_
     }
 }
 
+// The lane's place in its grid cell, for cutting its stencil to the cells
+// a neighbour can occupy.  A neighbour j of lane i has fl(d^2) < fl(h^2),
+// so its true distance is below h (1 + 2^-21), i.e. below rho = h / H cell
+// edges.  The cell offset d along an axis lies at least gap(d) cell edges
+// away (d - f for d > 0, -d - 1 + f for d < 0, 0 for d = 0, f = the lane's
+// fraction of its cell), so a cell whose squared gaps add up to rho^2 or
+// more holds no neighbour.  f comes from the same double expression as the
+// cell index, and the float work around it errs by far less than the slack
+// taken off every gap (2^-12 edges) and added to the reach (2^-10 of
+// rho^2): the cut drops only candidates whose float squared distance
+// exceeds h^2.
+struct LaneTrim {
+    float fx, fy, fz;  // fraction of the cell, clamped to [0, 1]
+    float r2;          // rho^2 (1 + 2^-10)
+};
+
+constexpr float kTrimSlack = 1.0f / 4096.0f;
+
+__device__ inline LaneTrim lane_trim(float4 p, uint32_t cc, const StepParams &P)
+{
+    auto frac = [&](float x, double o, int g) {
+        const double q = ((double)x - o) * P.grid_inv - (double)g;
+        return fminf(fmaxf((float)q, 0.0f), 1.0f);  // NaN -> 0
+    };
+    LaneTrim t;
+    t.fx = frac(p.x, P.grid_o[0], cell_x(cc));
+    t.fy = frac(p.y, P.grid_o[1], cell_y(cc));
+    t.fz = frac(p.z, P.grid_o[2], cell_z(cc));
+    const double rh = (double)P.h * P.grid_inv;
+    t.r2 = (float)(rh * rh * (1.0 + 1.0 / 1024.0));
+    return t;
+}
+
+// gap(d) less the slack, at least 0
+__device__ inline float trim_gap(int d, float f)
+{
+    const float g = d > 0 ? (float)d - f : (d < 0 ? (float)(-d - 1) + f : 0.0f);
+    return fmaxf(g - kTrimSlack, 0.0f);
+}
+
+// for_each_row cut by the lane's trim: row (y, z) is skipped when its
+// gaps alone reach rho, else its x-run is cut to the cells within the
+// remaining chord.  Calls row(b, e) for every non-empty row, in
+// for_each_row's order, so the passing candidates come in the same order
+// as on the whole stencil.  The bounds of the next row are loaded while the
+// current row runs.
+template <class Row>
+__device__ inline void for_each_cut_row(const uint32_t *__restrict__ cstart, uint32_t cc, const LaneTrim &T,
+                                        const StepParams &P, Row row)
+{
+    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc), R = P.reach;
+    const int y0 = max(gy - R, 0), y1 = min(gy + R, P.ny - 1);
+    const int z0 = max(gz - R, 0), z1 = min(gz + R, P.nz - 1);
+    const int nrows = (y1 - y0 + 1) * (z1 - z0 + 1);
+    auto bounds = [&](int y, int z, uint32_t &b, uint32_t &e) {
+        const float ay = trim_gap(y - gy, T.fy), az = trim_gap(z - gz, T.fz);
+        const float rem = T.r2 - (ay * ay + az * az);
+        b = e = 0u;
+        if (rem > 0.0f) {
+            const float w = __builtin_amdgcn_sqrtf(rem) + 2.0f * kTrimSlack;
+            const int x0 = max(gx + max((int)floorf(T.fx - w), -R), 0);
+            const int x1 = min(gx + min((int)floorf(T.fx + w), R), P.nx - 1);
+            if (x0 <= x1) {
+                const uint32_t r0 = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
+                b = cstart[r0 + (uint32_t)x0];
+                e = cstart[r0 + (uint32_t)x1 + 1u];
+            }
+        }
+    };
+    int y = y0, z = z0;
+    uint32_t b, e;
+    bounds(y, z, b, e);
+    for (int k = 0; k < nrows; ++k) {
+        if (++y > y1) {
+            y = y0;
+            ++z;
+        }
+        uint32_t bn = 0, en = 0;
+        if (k + 1 < nrows)
+            bounds(y, z, bn, en);
+        if (b < e)
+            row(b, e);
+        b = bn;
+        e = en;
+    }
+}
+
 // for_each_row restricted to the positions [lo, hi) of the lane's own
 // candidate sequence (its rows in order, slots ascending).
 template <class Row>
@@ -711,8 +798,12 @@ __device__ inline uint32_t row_base(const StepParams &P, int y, int z)
 // float squared distance is above h^2 and it never passes: both walks
 // visit a lane's passing candidates in the same order (rows z-major, then
 // y, slots ascending) and give bit-identical sums.
+// own: the lane's own candidate count when this computed it (the
+// comparison needs it unless all lanes share one cell), else kNoOwn.
+constexpr uint32_t kNoOwn = 0xFFFFFFFFu;
+
 __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t cc, const StepParams &P,
-                                  UnionRows &U, uint32_t &count)
+                                  UnionRows &U, uint32_t &count, uint32_t &own)
 {
     if (P.lane_walk)
         return false;
@@ -752,7 +843,7 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
     count = __builtin_amdgcn_readfirstlane(un);
     if (one)
         return true;
-    uint32_t own = 0;
+    own = 0;
     for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
     const uint32_t own_max = (uint32_t)wave_max_i((int)own);
     return count <= 3ull * own_max;
@@ -776,7 +867,8 @@ __device__ inline bool walk_of(const PartPlan &pp, uint32_t wt, const uint32_t *
         count = (uint32_t)__builtin_amdgcn_readfirstlane(b.z);
         return __builtin_amdgcn_readfirstlane(b.w) != 0;
     }
-    return plan_union(cstart, cc, P, U, count) && !__any(slow);
+    uint32_t own = kNoOwn;
+    return plan_union(cstart, cc, P, U, count, own) && !__any(slow);
 }
 
 // Length of the lane's own candidate sequence (its (2R + 1)^3 stencil).
@@ -928,8 +1020,8 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
         const StepParams &P = *L.P;
         const uint32_t cc = cellcB[L.slot];
         UnionRows U{0, 0, 0, 0, 0, 0};
-        uint32_t ucount = 0, len;
-        const bool use_union = plan_union(cstart, cc, P, U, ucount) && !__any((cc & kSlowBit) != 0u);
+        uint32_t ucount = 0, len, own = kNoOwn;
+        const bool use_union = plan_union(cstart, cc, P, U, ucount, own) && !__any((cc & kSlowBit) != 0u);
         if (use_union) {
             len = ucount;
         } else {
@@ -938,8 +1030,10 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
             // so they count three times toward the part size; with slow
             // lanes (m_ij counted from the 27 buckets per neighbour, the
             // wave waiting on them) more again.
-            uint32_t own = 0;
-            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+            if (__builtin_amdgcn_readfirstlane(own) == kNoOwn) {  // wave-uniform
+                own = 0;
+                for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+            }
             const uint32_t weight = __any((cc & kSlowBit) != 0u) ? kSlowWalkWeight : kLaneWalkWeight;
             len = weight * (uint32_t)wave_max_i((int)own);
             if (active)
@@ -1238,8 +1332,11 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             // Once a lane has more neighbours than a list holds, the force
             // pass walks this task anyway: the wave stops recording (and
             // only sums) for the rest of its rows.
+            // The cut stencil: it drops only candidates that fail, each of
+            // which adds +0 below, so the sums equal the whole stencil's.
             bool listing = true;
-            for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+            const LaneTrim T = lane_trim(pi, cc, P);
+            for_each_cut_row(cstart, cc, T, P, [&](uint32_t b, uint32_t e) {
                 for (uint32_t g = b; g < e; g += 64u) {
                     const uint32_t ge = min(g + 64u, e);
                     uint64_t mask = 0;

@@ -196,3 +196,64 @@ def test_walks_and_forces_at_1m_sampled(sph, orc32):
             _check_forces(sph, orc32, sc, st["pos"], st["vel"], ids, ost, label)
         finally:
             ost.close()
+
+
+def _corner_pairs(h, scale, npairs, seed):
+    """Pairs whose first particle sits just inside the low corner of a grid
+    cell and whose second lies just under h from it in a random direction
+    of the negative octant (some directions in a coordinate plane or along
+    an axis), inside the box [-0.3, 0.3]^3.  The first particle's cell is
+    then the farthest cell of the second's stencil that can hold a
+    neighbour, the case the per-lane stencil cut decides at its margin.
+    Returns positions and the grid (origin, edge, cells per axis) a scene
+    of them gets."""
+    H = h * (1.0 + 2.0 ** -10) * scale
+    o = -0.3 - H
+    dims = int(np.floor(0.6 / H)) + 3
+    R = int(np.ceil(1.0 / scale - 1e-9))
+    rng = np.random.default_rng(seed)
+    pts = []
+    for _ in range(npairs):
+        k = rng.integers(R + 3, dims - R - 3, size=3)
+        j = (o + k * H + 1e-6).astype(F32)
+        u = rng.uniform(0.05, 1.0, size=3)
+        u[rng.random(3) < 0.2] = 0.0
+        if not u.any():
+            u[0] = 1.0
+        u /= np.linalg.norm(u)
+        i = (j.astype(np.float64) - h * (1.0 - rng.uniform(0.0, 3e-7)) * u).astype(F32)
+        pts += [j, i]
+    return np.array(pts, dtype=F32), (o, H, dims)
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.5, 1.0 / 3.0])
+@pytest.mark.parametrize("lane_walk", [True, False])
+def test_stencil_cut_keeps_pairs_at_the_corner_margin(sph, orc32, scale, lane_walk):
+    """The per-lane walk that records neighbour lists visits only the cells
+    of its stencil within h of the lane (rows beyond h skipped, x-runs cut
+    to the chord): pairs at distance just under h reaching exactly into a
+    cell's corner are all found -- counts bit-exact and lists equal to the
+    oracle's -- on grids of reach 1, 2 and 3, with every wave on the
+    per-lane walk or with the plan's choice."""
+    h = 0.05
+    pos, (o, H, dims) = _corner_pairs(h, scale, 1500, seed=int(77 + 10 * scale))
+    n = len(pos)
+    assert dims ** 3 <= max(64 * n, 1 << 16), "premise: the grid is not coarsened"
+    assert pos.min() > -0.3 and pos.max() < 0.3
+    d = pos[0::2] - pos[1::2]
+    d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    assert (d2 < F32(h) * F32(h)).sum() > 200, "premise: most pairs are neighbours"
+    sc = sph_scenes.Scene("corner-pairs", pos, np.zeros_like(pos), 0.02 * (h / 0.15) ** 3, 1000.0, 1.0, 1.04,
+                          h, -9.8, 0.2, 1e-4, (-0.3, -0.3, -0.3), (0.3, 0.3, 0.3), steps=1)
+    with sph.Context(n, sph.settings(*sc.settings_args()), sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset),
+                     sc.dt, cell_scale=scale, lane_walk=lane_walk) as ctx:
+        ctx.upload(sc.pos, sc.vel)
+        dump = ctx.walk_dump(lists=True)
+    if lane_walk:
+        assert not dump["union"].any()
+    ids = np.arange(n)
+    ost = OracleState(orc32, sc, sc.pos, sc.vel)
+    try:
+        _check_dump(dump, ost.rows(ids), ids, f"corner pairs, scale {scale:.3f}")
+    finally:
+        ost.close()

@@ -198,6 +198,69 @@ def test_grid_cells_of_neighbours_are_adjacent(reach):
     assert checked > 1000
 
 
+@pytest.mark.parametrize("reach", [1, 2, 3])
+def test_stencil_cut_keeps_every_neighbour(reach):
+    """The per-lane stencil cut's margin: a lane at fraction f of its cell
+    skips row (dy, dz) when gy^2 + gz^2 >= r2 and keeps the x-cells
+    [floor(fx - w), floor(fx + w)] with w = sqrt(r2 - gy^2 - gz^2) +
+    2^-11, where gap(d) = d - f (d > 0), -d - 1 + f (d < 0) or 0, less
+    2^-12, r2 = (h / H)^2 (1 + 2^-10), all in float32 as the kernels
+    evaluate them.  For 60,000 pairs with fl(d^2) < fl(h*h), the first
+    placed just inside a random cell corner and the second just under h
+    away in a random direction, the first's cell is never cut from the
+    second's stencil."""
+    h = F32(0.03)
+    H = float(h) * (1.0 + 2.0 ** -10) / reach
+    origin = np.array([-2.0 - H, -1.0 - H, -1.0 - H])
+    inv = 1.0 / H
+    dims = (140 * reach, 70 * reach, 70 * reach)
+    rng = np.random.default_rng(17 + reach)
+    m = 60000
+    k = np.stack([rng.integers(reach + 2, 40 * reach, size=m) for _ in range(3)], axis=1)
+    j = (origin + k * H + rng.uniform(0, 2e-7, size=(m, 3)) * rng.choice([0.0, 1.0], size=(m, 3))).astype(F32)
+    u = rng.normal(size=(m, 3))
+    u[rng.random((m, 3)) < 0.15] = 0.0
+    u[~u.any(axis=1), 0] = 1.0
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    i = (j.astype(np.float64) + float(h) * (1.0 - rng.uniform(0, 4e-7, size=(m, 1))) * u).astype(F32)
+    dv = j - i
+    d2 = (dv[:, 0] * dv[:, 0] + dv[:, 1] * dv[:, 1]) + dv[:, 2] * dv[:, 2]
+    nb = d2 < h * h
+    assert nb.sum() > m // 4
+    gi, gj = brute.grid_coords(i, origin, inv, dims), brute.grid_coords(j, origin, inv, dims)
+    off = gj - gi
+    assert np.all(np.abs(off[nb]) <= reach)
+    f = np.clip((((i.astype(np.float64) - origin) * inv) - gi).astype(F32), F32(0), F32(1))
+    slack = F32(2.0 ** -12)
+    r2 = F32((float(h) * inv) ** 2 * (1.0 + 2.0 ** -10))
+
+    def gap(d, fr):
+        g = np.where(d > 0, d.astype(F32) - fr, np.where(d < 0, (-d - 1).astype(F32) + fr, F32(0)))
+        return np.maximum(g - slack, F32(0)).astype(F32)
+
+    ay, az = gap(off[:, 1], f[:, 1]), gap(off[:, 2], f[:, 2])
+    rem = (r2 - (ay * ay + az * az)).astype(F32)
+    kept_row = rem > 0
+    w = (np.sqrt(np.maximum(rem, F32(0))) + F32(2) * slack).astype(F32)
+    lo = np.maximum(np.floor(f[:, 0] - w).astype(np.int64), -reach)
+    hi = np.minimum(np.floor(f[:, 0] + w).astype(np.int64), reach)
+    kept = kept_row & (off[:, 0] >= lo) & (off[:, 0] <= hi)
+    assert np.all(kept[nb]), f"{(~kept[nb]).sum()} neighbours cut"
+    # the pairs sit at the margin: measuring the gap from the far side of
+    # the lane's cell (d - 1 + f for d > 0) would cut neighbours
+    bad = np.where(off[:, 1] > 0, np.maximum((off[:, 1] - 1).astype(F32) + f[:, 1] - slack, F32(0)), ay)
+    assert ((r2 - (bad * bad + az * az)) <= 0)[nb].any() or (off[nb, 1] <= 0).all()
+    # and the cut does cut: of the (2R + 1)^3 cells around lanes placed
+    # uniformly in their cell, about a quarter (R = 1) to a third (R = 3)
+    # are dropped
+    fr = rng.uniform(0, 1, size=(4000, 3)).astype(F32)
+    o = np.arange(-reach, reach + 1)
+    dx, dy, dz = [a.ravel() for a in np.meshgrid(o, o, o, indexing="ij")]
+    gx = gap(dx[None, :], fr[:, :1])
+    g2 = gx * gx + gap(dy[None, :], fr[:, 1:2]) ** 2 + gap(dz[None, :], fr[:, 2:3]) ** 2
+    assert (g2 < r2).mean() < [0.78, 0.69, 0.65][reach - 1]
+
+
 def test_invalid_arguments_fail_without_touching_a_device(sph):
     lib = sph.load()
     s = sph.settings()
