@@ -97,11 +97,12 @@ typedef struct sph_scene_desc_v1 {
 #define SPH_SCENE_LANE_WALK 2u
 
 /* The cell size follows the state: before the scene's first step and every
- * 64 steps after that, the candidates per neighbour of a sample of its
- * particles choose cells of h / 2 (a state uniform at the scale of h: about
- * 40% fewer candidates per particle) or of h (a state clumped below h,
- * where finer cells only add stencil rows).  Scenes of fewer than 131,072
- * particles keep cells of h.  The choice depends only on the scene's own
+ * 64 steps after that (every 256 while the scene is clumped well below h on
+ * cells of h), the candidates per neighbour of a sample of its particles
+ * choose cells of h / 2 (a state uniform at the scale of h: about 40% fewer
+ * candidates per particle) or of h (a state clumped below h, where finer
+ * cells only add stencil rows).  Scenes of fewer than 131,072 particles
+ * keep cells of h.  The choice depends only on the scene's own
  * state and step count, so the scene steps the same alone and in any batch;
  * results move by rounding when the grid changes. */
 #define SPH_SCENE_AUTO_GRID 4u

@@ -12,18 +12,27 @@ namespace sph {
 constexpr uint32_t kMaxParticles = 1u << 30;
 
 // Automatic cell size (SPH_SCENE_AUTO_GRID): before a scene's first step
-// and every kRegridEvery steps after that, the sampled candidates per
-// neighbour of its current state choose its cell scale: a state uniform at
-// the scale of h takes cells of h / 2 (about 40% fewer candidates), one
-// clumped below h takes cells of h (fewer stencil rows).  The choice
-// depends on the scene's state and step count alone.
+// and every kRegridEvery steps after that (kRegridClumped below), the
+// sampled candidates per neighbour of its current state choose its cell
+// scale: a state uniform at the scale of h takes cells of h / 2 (about 40%
+// fewer candidates), one clumped below h takes cells of h (fewer stencil
+// rows).  The choice depends on the scene's state and history alone.
 constexpr uint64_t kRegridEvery = 64;
-// Thresholds measured on the 1M-particle dam break: cells of h / 2 run its
-// steps faster while the ratio stays above ~6.5 at scale 1 (~3.8 at 1/2;
-// the jittered lattice start reads 7.2 and 4.2), slower after; smaller
-// lattices read less (their surfaces), an 8,000-particle cube 6.3.
-constexpr double kToFineRatio = 6.0;    // at cell scale 1: above this, go to 1/2
-constexpr double kToCoarseRatio = 3.5;  // at cell scale 1/2: below this, go to 1
+// Thresholds from same-state timings of both grids along the 1M-particle
+// dam break (20 steps from the states at steps 0, 64, 128, ...): cells of
+// h / 2 were faster at steps 0 and 64 (ratios 7.32 / 7.07 on cells of h,
+// 4.24 / 4.12 on cells of h / 2) and slower from step 128 on (6.62 / 3.90);
+// the two ratios of one state differ by a factor of about 1.7.  Smaller
+// lattices read less (their surfaces): a 52^3 one 7.2, an 8,000-particle
+// cube 6.3.
+constexpr double kToFineRatio = 6.85;  // at cell scale 1: above this, go to 1/2
+constexpr double kToCoarseRatio = 4.0;  // at cell scale 1/2: below this, go to 1
+// A scene on cells of h that read below kClumpedRatio (clumped well below
+// h; the dam break from step ~250 on) is evaluated every kRegridClumped
+// steps only: each evaluation sorts the state once more and waits for the
+// device, about 0.8 ms on the collapsed 1M-particle flow.
+constexpr double kClumpedRatio = 4.0;
+constexpr uint64_t kRegridClumped = 256;
 // Smaller scenes keep cells of h: on the batched workload (256 scenes of
 // 1k-64k particles) cells of h / 2 made the 500-step job 26% slower (eight
 // times the cells to scan, and no faster density pass at these sizes).
@@ -95,8 +104,8 @@ struct Batch {
     // the last step's (0 if there is none).
     int grid_stat(std::vector<double> &ratio, bool fresh);
     // Evaluates the automatic grid of the scenes whose step count is at a
-    // re-grid point (a multiple of kRegridEvery, the upload included) and
-    // re-grids those whose choice changed.
+    // re-grid point (a multiple of kRegridEvery, of kRegridClumped while
+    // clumped; the upload included) and re-grids those whose choice changed.
     int auto_grid();
     int grid_info(uint32_t s, float *cell_scale, double *cand_per_nbr);
     // Cells the in-cell sort left in arrival order (more than

@@ -465,7 +465,10 @@ int Batch::auto_grid()
 {
     int rc;
     auto due = [](const BatchScene &b) {
-        return (b.desc.flags & SPH_SCENE_AUTO_GRID) != 0u && b.steps % kRegridEvery == 0;
+        const double cur = b.scale > 0.0 ? b.scale : (b.desc.cell_scale > 0.0f ? (double)b.desc.cell_scale : 1.0);
+        const bool clumped = b.steps > 0 && cur >= 1.0 && b.last_ratio < kClumpedRatio;
+        return (b.desc.flags & SPH_SCENE_AUTO_GRID) != 0u &&
+               b.steps % (clumped ? kRegridClumped : kRegridEvery) == 0;
     };
     bool any = false;
     for (const BatchScene &b : scenes)

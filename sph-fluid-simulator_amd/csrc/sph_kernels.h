@@ -189,8 +189,11 @@ void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellA
 // Sampled candidates per neighbour of every scene (one wave task in
 // kGridStatEvery of each scene), from the current B order and cell ranges:
 // stat[2 s] = candidates, stat[2 s + 1] = neighbours of scene s (cleared
-// first).
+// first).  A lane with more than kGridStatTests candidates tests every
+// k-th of them (k = ceil(candidates / kGridStatTests)) and counts each
+// neighbour found k times.
 constexpr uint32_t kGridStatEvery = 16;
+constexpr uint32_t kGridStatTests = 256;
 void launch_grid_stat(hipStream_t st, const ParticleArrays &a, const CellArrays &c, const SceneSet &S,
                       unsigned long long *stat, uint32_t nscenes);
 // Drop-in output (single scene): order slots by the pre-step 16-bit bucket

@@ -1754,7 +1754,11 @@ void launch_force_integrate(hipStream_t st, const ParticleArrays &a, const CellA
 // wave adds its sums to stat[2 scene] and stat[2 scene + 1].  The ratio
 // tells a state uniform at the scale of h (a jittered lattice of spacing
 // h / 2 reads 7.2 at cell scale 1, 4.2 at cell scale 1/2) from one clumped
-// below it (near 1).
+// below it (near 1).  Lanes with more than kGridStatTests candidates (the
+// clumps of a collapsed flow, thousands each) test an evenly strided subset
+// and scale their count, so the sample costs a few hundred tests a lane
+// however dense the state; the uniform states the choice turns on have
+// fewer candidates and are counted exactly.
 __global__ __launch_bounds__(kBlock) void k_grid_stat(const float4 *__restrict__ pos,
                                                      const uint32_t *__restrict__ cellc,
                                                      const uint32_t *__restrict__ cstart, SceneSet S,
@@ -1774,13 +1778,19 @@ __global__ __launch_bounds__(kBlock) void k_grid_stat(const float4 *__restrict__
     if (active) {
         const float4 pi = pos[s];
         const float h2 = P.h2;
-        for_each_row(cstart, cellc[s], P, [&](uint32_t b, uint32_t e) {
-            cand += e - b;
-            for (uint32_t j = b; j < e; ++j) {
+        const uint32_t cc = cellc[s];
+        for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { cand += e - b; });
+        const uint32_t stride = max((cand + kGridStatTests - 1u) / kGridStatTests, 1u);
+        uint32_t q = 0;  // offset of the next tested candidate in the current row
+        for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) {
+            for (; q < e - b; q += stride) {
+                const uint32_t j = b + q;
                 float dx, dy, dz;
                 nbr += (j != s && dist2_ref(pi, pos[j], dx, dy, dz) < h2) ? 1u : 0u;
             }
+            q -= e - b;
         });
+        nbr *= stride;
     }
     unsigned long long c = cand, nb = nbr;
 #pragma unroll

@@ -239,9 +239,10 @@ def test_auto_grid_follows_the_state_and_keeps_batch_identity(sph, orc32):
     """SPH_SCENE_AUTO_GRID: a jittered lattice (uniform at the scale of h)
     moves to cells of h / 2 after its first step while a clumped scene
     stays on cells of h (premises, from the sampled candidates per
-    neighbour); over 70 steps (two re-grid points) each scene is
-    bit-identical alone and in a batch beside scenes without the flag; the
-    step after that matches the oracle fed the same state."""
+    neighbour); over 70 steps (re-grid points at steps 0 and 64) each scene
+    is bit-identical alone and in a batch beside scenes without the flag; a
+    step from the lattice's state on its fine grid (step 40) matches the
+    oracle fed the same state."""
     h = 0.03  # a 52^3 jittered lattice of spacing h / 2 (the dam break's start), above the size gate
     pos = sph_scenes.lattice((52, 52, 52), 0.5 * h, [-0.5, -0.9, -0.5], 0.01 * h, 3)
     lattice = sph_scenes.Scene("lattice-52", pos, np.zeros_like(pos), 1.6e-4, 1000.0,
@@ -278,7 +279,7 @@ def test_auto_grid_follows_the_state_and_keeps_batch_identity(sph, orc32):
                      sph.box(sc.box_lo, sc.box_hi, sc.restitution, sc.offset), sc.dt, auto_grid=True,
                      keep_force=True) as ctx:
         ctx.upload(sc.pos, sc.vel)
-        ctx.step(70)
+        ctx.step(40)
         assert ctx.grid_info()[0] == 0.5
         st = _by_id(ctx.download())
         ctx.step(1)
