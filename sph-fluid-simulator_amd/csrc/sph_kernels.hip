@@ -1307,27 +1307,34 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             // itself, adds +0, which leaves acc unchanged bit for bit (as
             // in the union walk), so both forms give the same sums.
             const bool any_slow = __any(slow);
-            auto fast = [&](uint32_t j, float4 q, uint32_t g) -> uint64_t {
+            uint32_t *lp = nl.list + 4ull * s;  // list entry nn (list_at's layout)
+            auto fast = [&](uint32_t j, float4 q) {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
                 const float t = fmaxf(h2 - d2, 0.0f);
                 acc = acc + mp6 * ((t * t) * t);
-                if (kCount)
-                    cnt += t > 0.0f ? 1u : 0u;
-                return t > 0.0f ? (1ull << (j - g)) : 0ull;
+                if (t > 0.0f) {
+                    if (kCount)
+                        ++cnt;
+                    if (nn < kListMax) {
+                        *lp = j;
+                        lp += (nn & 3u) == 3u ? 4ull * nl.stride - 3ull : 1ull;
+                    }
+                    ++nn;
+                }
             };
-            // candidates [j0, j1) of the segment starting at g
-            auto run_fast = [&](uint32_t j0, uint32_t j1, uint32_t g, uint64_t &mask) {
+            // candidates [j0, j1)
+            auto run_fast = [&](uint32_t j0, uint32_t j1) {
                 uint32_t j = j0;
                 for (; j + 4u <= j1; j += 4u) {  // four candidates in flight
                     const float4 q0 = pos[j], q1 = pos[j + 1u], q2 = pos[j + 2u], q3 = pos[j + 3u];
-                    mask |= fast(j, q0, g);
-                    mask |= fast(j + 1u, q1, g);
-                    mask |= fast(j + 2u, q2, g);
-                    mask |= fast(j + 3u, q3, g);
+                    fast(j, q0);
+                    fast(j + 1u, q1);
+                    fast(j + 2u, q2);
+                    fast(j + 3u, q3);
                 }
                 for (; j < j1; ++j)
-                    mask |= fast(j, pos[j], g);
+                    fast(j, pos[j]);
             };
             // Once a lane has more neighbours than a list holds, the force
             // pass walks this task anyway: the wave stops recording (and
@@ -1337,20 +1344,18 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             bool listing = true;
             const LaneTrim T = lane_trim(pi, cc, P);
             for_each_cut_row(cstart, cc, T, P, [&](uint32_t b, uint32_t e) {
+                if (!any_slow) {
+                    // the row split at the lane's own slot (when it holds
+                    // it), so no candidate needs a self test
+                    const uint32_t m = (s >= b && s < e) ? s : e;
+                    run_fast(b, m);
+                    run_fast(min(m + 1u, e), e);
+                    return;
+                }
                 for (uint32_t g = b; g < e; g += 64u) {
                     const uint32_t ge = min(g + 64u, e);
                     uint64_t mask = 0;
                     uint32_t j = g;
-                    if (!any_slow) {
-                        // the segment split at the lane's own slot (when it
-                        // holds it), so no candidate needs a self test
-                        const uint32_t m = (s >= g && s < ge) ? s : ge;
-                        run_fast(g, m, g, mask);
-                        run_fast(min(m + 1u, ge), ge, g, mask);
-                        j = ge;
-                        if (!listing)
-                            continue;
-                    }
                     for (; j + 1u < ge; j += 2u) {  // two candidates in flight
                         const float4 q0 = pos[j];
                         const float4 q1 = pos[j + 1u];
@@ -1373,6 +1378,8 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                     listing = !__any(nn > kListMax);
                 }
             });
+            if (!any_slow)
+                listing = !__any(nn > kListMax);
             listed = listing ? nn : kNoList;
         } else {
             uint32_t lo = 0, hi = 0xFFFFFFFFu;
