@@ -798,6 +798,32 @@ __device__ inline uint32_t row_base(const StepParams &P, int y, int z)
 // float squared distance is above h^2 and it never passes: both walks
 // visit a lane's passing candidates in the same order (rows z-major, then
 // y, slots ascending) and give bit-identical sums.
+// Length of the lane's own (2R + 1)^3 stencil, the sum of its rows'
+// lengths as for_each_row visits them; the bounds of one z-layer's rows are
+// loaded together (all in flight at once) instead of a row ahead.
+__device__ inline uint32_t stencil_length(const uint32_t *__restrict__ cstart, uint32_t cc, const StepParams &P)
+{
+    const int gx = cell_x(cc), gy = cell_y(cc), gz = cell_z(cc), R = P.reach;
+    const int x0 = max(gx - R, 0), x1 = min(gx + R, P.nx - 1);
+    const int y0 = max(gy - R, 0), y1 = min(gy + R, P.ny - 1);
+    const int z0 = max(gz - R, 0), z1 = min(gz + R, P.nz - 1);
+    uint32_t own = 0;
+    for (int z = z0; z <= z1; ++z) {
+        uint32_t b[2 * kMaxReach + 1], e[2 * kMaxReach + 1];
+#pragma unroll
+        for (int k = 0; k < 2 * kMaxReach + 1; ++k) {
+            const int y = min(y0 + k, y1);  // rows past y1 repeat y1 and count 0
+            const uint32_t r0 = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
+            b[k] = cstart[r0 + (uint32_t)x0];
+            e[k] = cstart[r0 + (uint32_t)x1 + 1u];
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * kMaxReach + 1; ++k)
+            own += (y0 + k <= y1) ? e[k] - b[k] : 0u;
+    }
+    return own;
+}
+
 // own: the lane's own candidate count when this computed it (the
 // comparison needs it unless all lanes share one cell), else kNoOwn.
 constexpr uint32_t kNoOwn = 0xFFFFFFFFu;
@@ -843,8 +869,7 @@ __device__ inline bool plan_union(const uint32_t *__restrict__ cstart, uint32_t 
     count = __builtin_amdgcn_readfirstlane(un);
     if (one)
         return true;
-    own = 0;
-    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
+    own = stencil_length(cstart, cc, P);
     const uint32_t own_max = (uint32_t)wave_max_i((int)own);
     return count <= 3ull * own_max;
 }
@@ -877,9 +902,7 @@ __device__ inline uint32_t own_length(const PartPlan &pp, uint32_t s, const uint
 {
     if (pp.own)
         return pp.own[s];
-    uint32_t own = 0;
-    for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
-    return own;
+    return stencil_length(cstart, cc, P);
 }
 
 // Scalar union walk: the positions [lo, hi) of the union's flattened
@@ -1030,10 +1053,8 @@ __global__ __launch_bounds__(kPlanBlock) void k_plan_parts(const uint32_t *__res
             // so they count three times toward the part size; with slow
             // lanes (m_ij counted from the 27 buckets per neighbour, the
             // wave waiting on them) more again.
-            if (__builtin_amdgcn_readfirstlane(own) == kNoOwn) {  // wave-uniform
-                own = 0;
-                for_each_row(cstart, cc, P, [&](uint32_t b, uint32_t e) { own += e - b; });
-            }
+            if (__builtin_amdgcn_readfirstlane(own) == kNoOwn)  // wave-uniform
+                own = stencil_length(cstart, cc, P);
             const uint32_t weight = __any((cc & kSlowBit) != 0u) ? kSlowWalkWeight : kLaneWalkWeight;
             len = weight * (uint32_t)wave_max_i((int)own);
             if (active)
