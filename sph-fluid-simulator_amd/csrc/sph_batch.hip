@@ -846,8 +846,8 @@ int Batch::walk_dump(uint32_t s, uint32_t *count, uint32_t *walk, uint32_t *list
         e = hipMemcpyAsync(off.data(), pp.off + t0, 4ull * (nt + 1), hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(wk.data(), pp.walk + 2ull * t0, sizeof(int4) * 2ull * nt, hipMemcpyDeviceToHost, stream);
-    for (uint32_t g = 0; list && g < kListMax / 4u && e == hipSuccess; ++g)  // groups of four entries
-        e = hipMemcpyAsync(lst.data() + 4ull * g * n, nl.list + list_at(4u * g, p0, nl.stride), 16ull * n,
+    for (uint32_t k = 0; list && k < kListMax && e == hipSuccess; ++k)  // entry k of every slot
+        e = hipMemcpyAsync(lst.data() + list_at(k, 0, n), nl.list + list_at(k, p0, nl.stride), 4ull * n,
                            hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(stream);

@@ -143,15 +143,15 @@ struct NeighborDump {
 // ascending) when there are at most kListMax of them; the force pass then
 // visits exactly those slots instead of filtering the stencil again.  The
 // order is the per-lane walk's, so the force sums are the same bit for bit.
-// Entries go in groups of four: entry k of slot s lives at
-// list[4 ((k / 4) stride + s) + k % 4], so one 16-byte load gives a lane
-// four neighbours and a wave's loads of one group are contiguous.
+// Entry k of slot s lives at list[k stride + s]: the lanes of a wave
+// that append their k-th neighbour together write one contiguous run, so
+// list lines are written whole.
 constexpr uint32_t kListMax = 64;
 constexpr uint32_t kNoList = 0xFFFFFFFFu;  // count of a slot without a list
 
 __host__ __device__ inline size_t list_at(uint32_t k, uint32_t s, uint32_t stride)
 {
-    return 4ull * ((size_t)(k >> 2) * stride + s) + (k & 3u);
+    return (size_t)k * stride + s;
 }
 
 struct NeighborLists {

@@ -1328,7 +1328,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             // itself, adds +0, which leaves acc unchanged bit for bit (as
             // in the union walk), so both forms give the same sums.
             const bool any_slow = __any(slow);
-            uint32_t *lp = nl.list + 4ull * s;  // list entry nn (list_at's layout)
+            uint32_t *lp = nl.list + s;  // list entry nn (list_at's layout)
             auto fast = [&](uint32_t j, float4 q) {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
@@ -1339,7 +1339,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                         ++cnt;
                     if (nn < kListMax) {
                         *lp = j;
-                        lp += (nn & 3u) == 3u ? 4ull * nl.stride - 3ull : 1ull;
+                        lp += nl.stride;
                     }
                     ++nn;
                 }
@@ -1642,16 +1642,20 @@ __global__ __launch_bounds__(BLOCK, 5) void k_force_integrate(
         uint32_t ucount = 0;
         if (use_list) {
             const uint32_t groups = ((uint32_t)wave_max_i((int)listed) + 3u) >> 2;
-            const uint4 *quads = reinterpret_cast<const uint4 *>(nl.list) + s;
+            const uint32_t *ent = nl.list + s;
             const size_t stride = nl.stride;
-            // groups <= kListMax / 4, so every group loaded lies inside the
+            // groups <= kListMax / 4, so every entry loaded lies inside the
             // list array; entries a lane has not written are never used
-            uint4 next = quads[0];
+            auto group = [&](uint32_t g) {
+                const uint32_t *e0 = ent + (size_t)(4u * g) * stride;
+                return make_uint4(e0[0], e0[stride], e0[2 * stride], e0[3 * stride]);
+            };
+            uint4 next = group(0);
             for (uint32_t g = 0; g < groups; ++g) {
                 const uint32_t k = 4u * g;
                 const uint4 cur = next;
                 if (g + 1u < groups)
-                    next = quads[(size_t)(g + 1u) * stride];
+                    next = group(g + 1u);
                 const uint32_t j0 = k < listed ? cur.x : s, j1 = k + 1u < listed ? cur.y : s;
                 const uint32_t j2 = k + 2u < listed ? cur.z : s, j3 = k + 3u < listed ? cur.w : s;
                 const float4 q0 = posR[j0], w0 = velR[j0], q1 = posR[j1], w1 = velR[j1];
