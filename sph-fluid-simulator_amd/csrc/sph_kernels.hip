@@ -812,14 +812,16 @@ __device__ inline uint32_t stencil_length(const uint32_t *__restrict__ cstart, u
         uint32_t b[2 * kMaxReach + 1], e[2 * kMaxReach + 1];
 #pragma unroll
         for (int k = 0; k < 2 * kMaxReach + 1; ++k) {
-            const int y = min(y0 + k, y1);  // rows past y1 repeat y1 and count 0
-            const uint32_t r0 = P.cell_base + (uint32_t)P.nx * ((uint32_t)y + (uint32_t)P.ny * (uint32_t)z);
-            b[k] = cstart[r0 + (uint32_t)x0];
-            e[k] = cstart[r0 + (uint32_t)x1 + 1u];
+            b[k] = e[k] = 0u;
+            if (y0 + k <= y1) {  // 2R + 1 rows (fewer at the grid's border)
+                const uint32_t r0 = P.cell_base + (uint32_t)P.nx * ((uint32_t)(y0 + k) + (uint32_t)P.ny * (uint32_t)z);
+                b[k] = cstart[r0 + (uint32_t)x0];
+                e[k] = cstart[r0 + (uint32_t)x1 + 1u];
+            }
         }
 #pragma unroll
         for (int k = 0; k < 2 * kMaxReach + 1; ++k)
-            own += (y0 + k <= y1) ? e[k] - b[k] : 0u;
+            own += e[k] - b[k];
     }
     return own;
 }
@@ -1328,34 +1330,59 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             // itself, adds +0, which leaves acc unchanged bit for bit (as
             // in the union walk), so both forms give the same sums.
             const bool any_slow = __any(slow);
+            // On fine grids (R >= 2: rows of a few candidates) each
+            // neighbour is appended as it passes; on cells of h (longer
+            // rows, most candidates failing) a row's passes are collected
+            // in a bitmask and appended afterwards, which issues fewer
+            // stores.  The order is the same.
+            const bool direct = P.reach >= 2;
             uint32_t *lp = nl.list + s;  // list entry nn (list_at's layout)
-            auto fast = [&](uint32_t j, float4 q) {
+            auto fast = [&](float4 q) -> bool {
                 float dx, dy, dz;
                 const float d2 = dist2_ref(pi, q, dx, dy, dz);
                 const float t = fmaxf(h2 - d2, 0.0f);
                 acc = acc + mp6 * ((t * t) * t);
-                if (t > 0.0f) {
-                    if (kCount)
-                        ++cnt;
-                    if (nn < kListMax) {
-                        *lp = j;
-                        lp += nl.stride;
-                    }
-                    ++nn;
-                }
+                if (kCount)
+                    cnt += t > 0.0f ? 1u : 0u;
+                return t > 0.0f;
             };
-            // candidates [j0, j1)
-            auto run_fast = [&](uint32_t j0, uint32_t j1) {
+            auto append = [&](uint32_t j) {
+                if (nn < kListMax) {
+                    *lp = j;
+                    lp += nl.stride;
+                }
+                ++nn;
+            };
+            // candidates [j0, j1), appended as they pass
+            auto run_direct = [&](uint32_t j0, uint32_t j1) {
                 uint32_t j = j0;
                 for (; j + 4u <= j1; j += 4u) {  // four candidates in flight
                     const float4 q0 = pos[j], q1 = pos[j + 1u], q2 = pos[j + 2u], q3 = pos[j + 3u];
-                    fast(j, q0);
-                    fast(j + 1u, q1);
-                    fast(j + 2u, q2);
-                    fast(j + 3u, q3);
+                    if (fast(q0))
+                        append(j);
+                    if (fast(q1))
+                        append(j + 1u);
+                    if (fast(q2))
+                        append(j + 2u);
+                    if (fast(q3))
+                        append(j + 3u);
                 }
                 for (; j < j1; ++j)
-                    fast(j, pos[j]);
+                    if (fast(pos[j]))
+                        append(j);
+            };
+            // candidates [j0, j1) of the segment starting at g, into its mask
+            auto run_mask = [&](uint32_t j0, uint32_t j1, uint32_t g, uint64_t &mask) {
+                uint32_t j = j0;
+                for (; j + 4u <= j1; j += 4u) {
+                    const float4 q0 = pos[j], q1 = pos[j + 1u], q2 = pos[j + 2u], q3 = pos[j + 3u];
+                    mask |= fast(q0) ? 1ull << (j - g) : 0ull;
+                    mask |= fast(q1) ? 2ull << (j - g) : 0ull;
+                    mask |= fast(q2) ? 4ull << (j - g) : 0ull;
+                    mask |= fast(q3) ? 8ull << (j - g) : 0ull;
+                }
+                for (; j < j1; ++j)
+                    mask |= fast(pos[j]) ? 1ull << (j - g) : 0ull;
             };
             // Once a lane has more neighbours than a list holds, the force
             // pass walks this task anyway: the wave stops recording (and
@@ -1365,18 +1392,25 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
             bool listing = true;
             const LaneTrim T = lane_trim(pi, cc, P);
             for_each_cut_row(cstart, cc, T, P, [&](uint32_t b, uint32_t e) {
-                if (!any_slow) {
+                if (!any_slow && direct) {
                     // the row split at the lane's own slot (when it holds
                     // it), so no candidate needs a self test
                     const uint32_t m = (s >= b && s < e) ? s : e;
-                    run_fast(b, m);
-                    run_fast(min(m + 1u, e), e);
+                    run_direct(b, m);
+                    run_direct(min(m + 1u, e), e);
                     return;
                 }
                 for (uint32_t g = b; g < e; g += 64u) {
                     const uint32_t ge = min(g + 64u, e);
                     uint64_t mask = 0;
                     uint32_t j = g;
+                    if (!any_slow) {
+                        // the segment split at the lane's own slot
+                        const uint32_t m = (s >= g && s < ge) ? s : ge;
+                        run_mask(g, m, g, mask);
+                        run_mask(min(m + 1u, ge), ge, g, mask);
+                        j = ge;
+                    }
                     for (; j + 1u < ge; j += 2u) {  // two candidates in flight
                         const float4 q0 = pos[j];
                         const float4 q1 = pos[j + 1u];
@@ -1399,7 +1433,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                     listing = !__any(nn > kListMax);
                 }
             });
-            if (!any_slow)
+            if (!any_slow && direct)
                 listing = !__any(nn > kListMax);
             listed = listing ? nn : kNoList;
         } else {
