@@ -46,6 +46,7 @@ struct BatchScene {
     uint64_t steps = 0;  // steps since the upload
     double scale = 0.0;  // cell scale of the current grid (0: desc.cell_scale)
     double last_ratio = 0.0;  // candidates per neighbour at the last evaluation
+    uint64_t evaluated = ~0ull;  // step count of the last evaluation (the upload's and the first step's are one)
 };
 
 int validate_desc(const sph_scene_desc_v1 &d);

@@ -292,6 +292,7 @@ int Batch::upload(uint32_t s, const float *pos, const float *vel)
     b.steps = 0;
     b.scale = 0.0;
     b.last_ratio = 0.0;
+    b.evaluated = ~0ull;
     layout_dirty = true;
     keys_dirty = true;
     stat_ok = false;
@@ -467,7 +468,7 @@ int Batch::auto_grid()
     auto due = [](const BatchScene &b) {
         const double cur = b.scale > 0.0 ? b.scale : (b.desc.cell_scale > 0.0f ? (double)b.desc.cell_scale : 1.0);
         const bool clumped = b.steps > 0 && cur >= 1.0 && b.last_ratio < kClumpedRatio;
-        return (b.desc.flags & SPH_SCENE_AUTO_GRID) != 0u &&
+        return (b.desc.flags & SPH_SCENE_AUTO_GRID) != 0u && b.steps != b.evaluated &&
                b.steps % (clumped ? kRegridClumped : kRegridEvery) == 0;
     };
     bool any = false;
@@ -483,6 +484,7 @@ int Batch::auto_grid()
         if (!due(b))
             continue;
         b.last_ratio = ratio[k];
+        b.evaluated = b.steps;
         const double cur = b.scale > 0.0 ? b.scale : (b.desc.cell_scale > 0.0f ? (double)b.desc.cell_scale : 1.0);
         double want = cur;
         if (cur >= 1.0 && ratio[k] > kToFineRatio && b.desc.n >= kAutoGridMinParticles)
