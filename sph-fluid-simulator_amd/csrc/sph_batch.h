@@ -100,6 +100,8 @@ struct Batch {
     int step(int nsteps);
     // nsteps steps on the current layout (no grid decisions)
     int step_plain(int nsteps);
+    // Captures the step into the graph when there is none (use_graph).
+    int capture();
     // Sampled candidates per neighbour of every scene (ratio[s]): fresh =
     // from the cell order of the current state (sorted here), else from
     // the last step's (0 if there is none).

@@ -307,6 +307,8 @@ int Batch::upload(uint32_t s, const float *pos, const float *vel)
         return rc;
     if ((rc = prepare()))
         return rc;
+    if (use_graph && (rc = capture()))  // the step's graph too, outside the first step
+        return rc;
     e = hipStreamSynchronize(stream);
     if (e != hipSuccess)
         return hip_fail(e, "upload");
@@ -501,6 +503,24 @@ int Batch::auto_grid()
     return SPH_OK;
 }
 
+int Batch::capture()
+{
+    if (exec)
+        return SPH_OK;
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess)
+        return hip_fail(e, "hipStreamBeginCapture");
+    enqueue(nullptr);
+    e = hipStreamEndCapture(stream, &graph);
+    if (e == hipSuccess)
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph();
+        return hip_fail(e, "graph capture");
+    }
+    return SPH_OK;
+}
+
 int Batch::step_plain(int nsteps)
 {
     int rc;
@@ -511,19 +531,8 @@ int Batch::step_plain(int nsteps)
     if ((rc = prepare()))
         return rc;
     if (use_graph) {
-        if (!exec) {
-            hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
-            if (e != hipSuccess)
-                return hip_fail(e, "hipStreamBeginCapture");
-            enqueue(nullptr);
-            e = hipStreamEndCapture(stream, &graph);
-            if (e == hipSuccess)
-                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                drop_graph();
-                return hip_fail(e, "graph capture");
-            }
-        }
+        if ((rc = capture()))
+            return rc;
         for (int k = 0; k < nsteps; ++k) {
             hipError_t e = hipGraphLaunch(exec, stream);
             if (e != hipSuccess)
