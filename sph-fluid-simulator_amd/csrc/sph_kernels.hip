@@ -1321,9 +1321,8 @@ __global__ __launch_bounds__(BLOCK, 6) void k_density(
                     cand(j, p[0], self_tag);
             });
         } else if ((MODE == 0 || MODE == 3) && nl.list && nparts == 1u) {
-            // Per-lane walk that also records the lane's neighbours: each
-            // row is filtered in segments of up to 64 candidates into a
-            // bitmask, whose set bits are then appended in slot order.
+            // Per-lane walk that also records the lane's neighbours, in
+            // walk order (rows in order, slots ascending).
             uint32_t nn = 0;
             // Without slow lanes every neighbour counts once, and the sum
             // runs branch-free: a failing candidate, or the particle
